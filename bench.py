@@ -1,0 +1,220 @@
+#!/usr/bin/env python3
+"""fix eph throughput on one MI355X.
+
+Workload (default): fcc 80^3 cells = 2,048,000 atoms, a = 3.615 A, four
+species assigned equiatomic at random, 300 K, one 10 keV primary knock-on
+atom at the box centre, pair lj/cut (0.4 eV, 2.3 A, 5.5 A), synthetic .beta
+with rho(r) = A exp(-r/0.9) (A = 1.6/1.8/2.0/2.2, cut 4.5 A) and
+beta(rho) = B rho/(1 + 0.5 rho) (B = 0.015/0.018/0.02/0.025), 1000 knots,
+T_e grid 80^3 with a 3000 K Gaussian hot spot (sigma 10 A), C_e = 3e-6,
+kappa = 0.1, dt = 0.1 fs, FP64, full model (friction + random + FDM).
+
+A step is one MD step; the timed quantity is the fix's own work per step
+(initial_integrate + post_force + final_integrate + end_of_step, measured
+with HIP events on the fix's stream), as the reference's "Modify" time is.
+value = atoms x steps / fix seconds.  The pair force, ghost update and
+neighbour rebuilds run in the same loop on the device (rebuilds through the
+host) and are reported separately.
+"""
+import argparse
+import json
+import math
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "user-eph_amd"))
+
+METRIC = "fix-eph atom-steps/s, 2M atoms FP64 (% HBM roofline); ensemble replica-steps/s"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def write_inputs(tmp, cells, a, grid, species):
+    import numpy as np
+    from eph_amd import inputs
+    beta = os.path.join(tmp, "alloy.beta")
+    names = inputs.workload_beta(beta, species)
+    L = cells * a
+    d = L / grid
+    idx = np.indices((grid, grid, grid))[::-1].reshape(3, -1).T
+    c = (idx + 0.5) * d - 0.5 * L
+    T = 300.0 + 2700.0 * np.exp(-(c * c).sum(1) / (2 * 10.0 ** 2))
+    tin = os.path.join(tmp, "hotspot.T")
+    with open(tin, "w") as f:
+        f.write("# T_e grid: 3000 K Gaussian hot spot, sigma 10 A\n#\n#\n")
+        f.write(f"{grid} {grid} {grid} 1\n")
+        for _ in range(3):
+            f.write(f"0.0 {L!r}\n")
+        f.write("NULL\n")
+        f.write("\n".join(f"{i} {j} {k} {t!r} 0.0 1.0 3e-06 0.1 1 0"
+                          for (i, j, k), t in zip(idx.tolist(), T.tolist())))
+        f.write("\n")
+    return beta, names, tin
+
+
+def cpu_baseline(args_fix, species, snap, md, dt, nsteps):
+    """The oracle (scalar C++ restatement of FixEPH) on the same system."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import eph_oracle as eo
+    from eph_amd import inputs
+    fix = eo.Fix(args_fix, species, snap.boxlo, snap.boxhi, dt, inputs.BOLTZ, inputs.FTM2V,
+                 masses=[inputs.MASS_CU] * species)
+    md.lj(snap)
+    t_fix = 0.0
+    for s in range(1, nsteps + 1):
+        t0 = time.perf_counter()
+        fix.initial_integrate(snap)
+        t_fix += time.perf_counter() - t0
+        md.update_ghosts(snap)
+        t0 = time.perf_counter()
+        fix.post_force(snap, step=s)
+        fix.final_integrate(snap)
+        fix.end_of_step(snap, s)
+        t_fix += time.perf_counter() - t0
+        log(f"cpu baseline step {s}/{nsteps}: {t_fix:.1f} s")
+    return snap.nlocal * nsteps / t_fix, t_fix
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--cells", type=int, default=80)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        if int(os.environ.get("RANK", "0")) == 0:
+            print("bench.py: this build measures one GPU; the multi-GPU path (spatial domain "
+                  "decomposition with halo exchange) is not built, so --gpus above 1 is not run.",
+                  file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import eph_amd as ea
+    from eph_amd import inputs
+
+    cells, a, species, grid, dt = args.cells, 3.615, 4, args.cells, 1e-4
+    tmp = tempfile.mkdtemp(prefix="ephbench")
+    t0 = time.time()
+    md = ea.MD(cells, a=a, ntypes=species, seed=2024, T=300.0, pka_eV=1.0e4)
+    inf = md.info()
+    N = inf["nlocal"]
+    log(f"system: {N} atoms, {inf['nghost']} ghosts, {inf['nfix'] / N:.1f} list entries/atom "
+        f"({time.time() - t0:.1f} s)")
+    beta, names, tin = write_inputs(tmp, cells, a, grid, species)
+    fargs = inputs.fix_args(beta, names, seed=4242, flags="7", grid=(grid, grid, grid),
+                            T_infile=tin, C_e=3e-6, kappa=0.1)
+    fix = ea.FixEPH(fargs, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU] * species)
+    fix.init()
+    cpu = None
+    if not args.no_cpu_baseline:
+        # timed before the device run, while the harness lists still match snap0
+        log("cpu baseline ...")
+        cpu = cpu_baseline(fargs, species, md.snapshot(), md, dt, args.cpu_steps)
+    md.attach(fix)
+    g = fix.grid()
+    T0max = g.get("T").max()
+    log(f"attached ({time.time() - t0:.1f} s)")
+
+    step = 1
+    blowup_step = None
+
+    def advance(n, timed):
+        nonlocal step, blowup_step
+        res = dict(fix_ms=0.0, total_ms=0.0, pair_ms=0.0, rebuild_ms=0.0, rebuilds=0)
+        for _ in range(n):
+            try:
+                r = md.run(fix, 1, first_step=step)
+            except ea.EphError as e:
+                if e.code != -7:
+                    raise
+                r = dict(fix_ms=float("nan"), total_ms=0.0, pair_ms=0.0, rebuild_ms=0.0, rebuilds=0)
+                if blowup_step is None:
+                    blowup_step = step
+            for k in res:
+                res[k] += r[k]
+            if blowup_step is None:
+                T = g.get("T")
+                if not np.all(np.isfinite(T)) or T.max() > 2.0 * T0max:
+                    blowup_step = step
+            step += 1
+        return res
+
+    advance(args.warmup, False)
+    log(f"warmup done ({time.time() - t0:.1f} s)")
+    fix.timing(True)
+    res = advance(args.steps, True)
+    rep = fix.timing_report()
+    fix.timing(False)
+    log(f"timed {args.steps} steps ({time.time() - t0:.1f} s): {res}")
+    nact = fix.get("nactive").mean()
+    nlist = fix.get("nlist").mean()
+    fix_s = res["fix_ms"] / 1e3
+    value = N * args.steps / fix_s
+    # dominant kernel: force_prl stages 2+3 (friction + random forces)
+    prl_ms, prl_n = rep.get("prl_forces", (float("nan"), 1))
+    prl_avg = prl_ms / max(prl_n, 1)
+    bytes_prl = (200.0 + 4.0 * nact) * N
+    achieved = bytes_prl / (prl_avg * 1e-3) / 1e9
+    dw_ms, dw_n = rep.get("density_w", (float("nan"), 1))
+    bytes_dw = (132.0 + 4.0 * nlist + 4.0 * nact) * N
+    kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rep.items()}
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "atom-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["fix_ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"fcc{cells}_alloy{species}_cascade",
+            "atoms": N, "ghosts": fix.nghost, "species": species, "pka_eV": 1.0e4,
+            "list_entries_per_atom": nlist, "active_pairs_per_atom": nact,
+            "Te_grid": [grid] * 3, "dt_ps": dt, "flags": 7, "model": 4,
+            "md_ms_per_step": res["total_ms"] / args.steps,
+            "pair_ms_per_step": res["pair_ms"] / args.steps,
+            "host_rebuilds": res["rebuilds"], "host_rebuild_ms": res["rebuild_ms"],
+            "fdm_blowup_step": blowup_step,
+            "kernels": kernels,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_prl_forces",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "bytes_per_atom": 200.0 + 4.0 * nact,
+            "avg_launch_ms": prl_avg,
+            "density_w_GBs": bytes_dw / (dw_ms / max(dw_n, 1) * 1e-3) / 1e9,
+        },
+        "cpu_baseline": None,
+    }
+    if cpu is not None:
+        v_cpu, t_cpu = cpu
+        out["cpu_baseline"] = {"value": v_cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                               "sample": f"{args.cpu_steps} fix steps of the same {N}-atom system "
+                                         f"({t_cpu:.1f} s), scalar oracle on one host core"}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,208 @@
+/*
+ * eph_amd.h -- C ABI of the MI355X-native `fix eph` hot path.
+ *
+ * One opaque context per fix instance.  The context owns every device buffer
+ * (atoms, neighbour list, spline tables, T_e grid, scratch) and a HIP stream;
+ * the caller owns host arrays.  Every call returns an int status (0 = ok,
+ * negative = enum eph_status) and never aborts the host; the message of the
+ * last failure is available from eph_last_error().
+ *
+ * Two ways to drive the hooks:
+ *   host arrays  -- eph_initial_integrate / eph_post_force / eph_final_integrate /
+ *                   eph_end_of_step take LAMMPS-layout host arrays ([n][3]
+ *                   doubles), upload, run the kernels and download, like the
+ *                   reference GPU path (fix_eph_gpu.cpp:91-205).
+ *   device state -- eph_dev_* run the same kernels on the atoms already
+ *                   resident in the context (used by the device-resident MD
+ *                   harness of eph_md.h and by the benchmark).
+ *
+ * References are USER-EPH file:line.
+ */
+#ifndef EPH_AMD_H
+#define EPH_AMD_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct eph_context eph_context;
+typedef struct eph_grid eph_grid;
+
+enum eph_status {
+  EPH_OK = 0,
+  EPH_ERR_ARGS = -1,        /* bad argument / illegal fix command            */
+  EPH_ERR_FILE = -2,        /* file missing or unreadable                    */
+  EPH_ERR_FORMAT = -3,      /* malformed .beta / grid / parameter file       */
+  EPH_ERR_OOM = -4,         /* device allocation failed                      */
+  EPH_ERR_HIP = -5,         /* HIP runtime error                             */
+  EPH_ERR_STATE = -6,       /* call out of order (e.g. no atoms uploaded)    */
+  EPH_ERR_NONFINITE = -7,   /* T_e grid became non-finite (FDM blow-up)      */
+  EPH_ERR_UNSUPPORTED = -8, /* model other than 4 (PRL) / -1 (TESTING)       */
+  EPH_ERR_NO_DEVICE = -9    /* no HIP device                                 */
+};
+
+/* eph_flag bits, fix_eph.h:43-50 */
+enum eph_flag {
+  EPH_FRICTION = 0x01, EPH_RANDOM = 0x02, EPH_FDM = 0x04,
+  EPH_NOINT = 0x08, EPH_NOFRICTION = 0x10, EPH_NORANDOM = 0x20
+};
+
+/* hooks a host must call, fix_eph.cpp:293-302 (FixEPH::setmask) */
+enum eph_hook {
+  EPH_HOOK_INITIAL_INTEGRATE = 1, EPH_HOOK_POST_FORCE = 2,
+  EPH_HOOK_FINAL_INTEGRATE = 4, EPH_HOOK_END_OF_STEP = 8
+};
+
+/* ---------------------------------------------------------------- lifetime */
+
+/* FixEPH::FixEPH, fix_eph.cpp:61-241.  arg/narg are the fix command words
+ * exactly as LAMMPS passes them (arg[0] = fix ID ... arg[17+] = element per
+ * type).  boxlo/boxhi = domain->boxlo/boxhi, dt = update->dt, boltz and
+ * ftm2v = force->boltz / force->ftm2v, ntypes = atom->ntypes.  device = HIP
+ * device ordinal. */
+int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
+               const double boxlo[3], const double boxhi[3], double dt, double boltz,
+               double ftm2v, int device);
+/* FixEPH::~FixEPH, fix_eph.cpp:244-261 */
+void eph_destroy(eph_context* ctx);
+/* message of the last failure on ctx (or of the last failed eph_create when ctx is NULL) */
+const char* eph_last_error(const eph_context* ctx);
+
+/* FixEPH::init, fix_eph.cpp:263-287: rejects 2-d, non-periodic and triclinic
+ * boxes; *cutoff receives the r_cutoff the neighbour request must use. */
+int eph_init(eph_context* ctx, int dimension, int nonperiodic, int triclinic, double* cutoff);
+/* FixEPH::setmask, fix_eph.cpp:293-302: returns the eph_hook bits */
+int eph_setmask(void);
+/* FixEPH::reset_dt, fix_eph.cpp:909-916 */
+int eph_reset_dt(eph_context* ctx, double dt);
+/* atom->mass[1..ntypes] (element 0 unused, as in LAMMPS) */
+int eph_set_masses(eph_context* ctx, const double* mass, int ntypes);
+/* Fix::groupbit; atoms with (mask & groupbit) belong to the fix group */
+int eph_set_groupbit(eph_context* ctx, int groupbit);
+/* flags, model, seed, r_cutoff, rho_cutoff, eta_factor, dtf (read-only view) */
+int eph_get_params(const eph_context* ctx, int* flags, int* model, int* seed, double* r_cutoff,
+                   double* rho_cutoff, double* eta_factor, double* dtf);
+
+/* ------------------------------------------------------------------- atoms */
+
+/* atom->x, atom->v ([nall][3]), atom->type (1-based), atom->mask, atom->tag
+ * for nall = nlocal + nghost; replaces cpu_to_device_EPH_GPU of
+ * lib/eph_gpu.h:75 and the host staging of fix_eph_gpu.cpp:110-112. */
+int eph_set_atoms(eph_context* ctx, int nlocal, int nghost, const double* x, const double* v,
+                  const int* type, const int* mask, const int* tag);
+/* On one process the forward communication of fix_eph.cpp:951-1009 is
+ * ghost <- owner: owner[g] is the local index whose image is ghost nlocal+g,
+ * shift[g][3] the periodic offset (x_ghost = x_owner + shift). */
+int eph_set_ghost_map(eph_context* ctx, int nghost, const int* owner, const double* shift);
+/* NeighList numneigh/firstneigh (full list incl. ghosts) in CSR form:
+ * neighbours of i are idx[off[i] .. off[i+1]), each masked with neighmask
+ * (NEIGHMASK) as fix_eph.cpp:451-452 does.  Replaces transfer_neighbour_list
+ * of fix_eph_gpu.cpp:241-280 / EPH_GPU::grow_neigh of lib/eph_gpu.h:67. */
+int eph_set_neighbors(eph_context* ctx, int nlocal, const int* off, const int* idx, int neighmask);
+/* Inject the Gaussian noise xi ([nlocal][3]) used by the next post_force for
+ * group atoms, instead of drawing it; NULL returns to the built-in stream.
+ * Replaces the RanMars draws of fix_eph.cpp:854-862. */
+int eph_set_noise(eph_context* ctx, const double* xi);
+/* the built-in stream draws xi_i = N(0,1)^3 from Philox4x32-10 keyed by
+ * (seed, tag_i, ntimestep); this sets ntimestep for the next post_force. */
+int eph_set_timestep(eph_context* ctx, long long ntimestep);
+
+/* ------------------------------------------------------- hooks, host arrays */
+
+/* FixEPH::initial_integrate, fix_eph.cpp:305-328; x, v updated in place */
+int eph_initial_integrate(eph_context* ctx, double* x, double* v, const double* f);
+/* FixEPH::post_force, fix_eph.cpp:841-907 (+ calculate_environment :431-466,
+ * force_prl :687-837); x, v are the current [nall][3] arrays, f [nlocal][3]
+ * is updated in place (f += f_EPH, f += f_RNG per the flags). */
+int eph_post_force(eph_context* ctx, const double* x, const double* v, double* f);
+/* FixEPH::final_integrate, fix_eph.cpp:330-348 */
+int eph_final_integrate(eph_context* ctx, double* v, const double* f);
+/* FixEPH::end_of_step, fix_eph.cpp:350-429 (+ EPH_FDM::solve, eph_fdm.h:267-400) */
+int eph_end_of_step(eph_context* ctx, const double* x, const double* v, long long ntimestep);
+/* FixEPH::post_run, fix_eph.cpp:1019-1021: writes the T_e state file */
+int eph_post_run(eph_context* ctx);
+/* FixEPH::compute_vector, fix_eph.cpp:940-948: 0 -> Ee, 1 -> mean T_e */
+int eph_compute_vector(eph_context* ctx, int i, double* out);
+/* per-atom array of fix_eph.cpp:406-428: [nlocal][8] = rho, beta(rho), f_EPH, f_RNG */
+int eph_get_peratom(eph_context* ctx, double* array);
+/* internal per-atom state after post_force, for checking:
+ * which 0 rho [nall], 1 w [nall][3], 2 xi [nall][3], 3 f_EPH [nlocal][3],
+ * 4 f_RNG [nlocal][3], 5 alpha [nall], 6 active neighbours (r < r_cutoff)
+ * per local atom [nlocal], 7 list entries per local atom [nlocal] */
+int eph_get_state(eph_context* ctx, int which, double* out);
+/* energy deposited in the last end_of_step (E_local of fix_eph.cpp:357-402) */
+int eph_last_energy(eph_context* ctx, double* E_local);
+/* non-zero once get_beta/get_alpha saw rho > rho_cutoff (eph_beta.h:174-194) */
+int eph_rho_cutoff_warning(eph_context* ctx);
+
+/* ---------------------------------------------------- hooks, device-resident */
+
+int eph_dev_initial_integrate(eph_context* ctx);
+int eph_dev_post_force(eph_context* ctx);
+int eph_dev_final_integrate(eph_context* ctx);
+int eph_dev_end_of_step(eph_context* ctx, long long ntimestep);
+/* copy resident x, v ([nall][3]) / f ([nlocal][3]) back to the host */
+int eph_dev_get_atoms(eph_context* ctx, double* x, double* v, double* f);
+/* wait for all work queued on the context's stream */
+int eph_sync(eph_context* ctx);
+/* the context's HIP stream (hipStream_t), for timing with HIP events */
+void* eph_stream(eph_context* ctx);
+
+/* Per-kernel timing with HIP events on the context's stream (no counterpart
+ * in the reference; used to measure kernel durations).  eph_timing_report with
+ * k < 0 returns the number of kernels recorded; with k >= 0 it fills the name,
+ * summed milliseconds and launch count of kernel k. */
+int eph_timing_enable(eph_context* ctx, int on);
+int eph_timing_report(eph_context* ctx, int k, char* name, int namelen, double* total_ms,
+                      long long* count);
+
+/* ------------------------------------------------ host-side table checks */
+/* Host construction only, no device needed.
+ * EPH_Spline(dx, y) of eph_spline.h:31-132: coeffs [n][4] = a, b, c, d of the
+ * global-x cubic per knot. */
+int eph_spline_build(double dx, const double* y, int n, double* coeffs);
+/* EPH_Beta(file) of eph_beta.h:39-128: table `which` (0 rho(r), 1 rho(r^2),
+ * 2 beta(rho), 3 alpha(rho)) of element `elem`.  Returns the knot count (or a
+ * negative status); coeffs [n][4] is filled when non-NULL. */
+int eph_beta_table(const char* path, int elem, int which, double* coeffs, double* inv_dx,
+                   double* r_cutoff, double* rho_cutoff);
+
+/* --------------------------------------------------------------- T_e grid */
+/* EPH_FDM counterpart (eph_fdm.h); also usable standalone. */
+
+/* EPH_FDM(nx, ny, nz, box, T_e, C_e, rho_e, kappa_e), eph_fdm.h:28-46 */
+int eph_grid_create(eph_grid** out, int nx, int ny, int nz, const double box[6], double T_e,
+                    double C_e, double rho_e, double kappa_e, int device);
+/* EPH_FDM(const char*), eph_fdm.h:48-119 (grid file + optional parameter file) */
+int eph_grid_load(eph_grid** out, const char* path, int device);
+void eph_grid_destroy(eph_grid* g);
+const char* eph_grid_last_error(const eph_grid* g);
+/* the grid owned by a fix context (not to be destroyed by the caller) */
+eph_grid* eph_context_grid(eph_context* ctx);
+int eph_grid_set_dt(eph_grid* g, double dt);          /* eph_fdm.h:154-157 */
+int eph_grid_set_steps(eph_grid* g, int steps);       /* eph_fdm.h:159-162 */
+/* one cell's T, S, rho_e, C_e, kappa, flag (0 constant, 1 dynamic, 2 zero
+ * derivative) and T-dependence flag, as a grid-file line sets them */
+int eph_grid_set_cell(eph_grid* g, int i, int j, int k, double T, double S, double rho,
+                      double C, double kappa, int flag, int tdyn);
+/* EPH_FDM::insert_energy (eph_fdm.h:172-179) for n points, in array order */
+int eph_grid_insert_energy(eph_grid* g, int n, const double* x, const double* y,
+                           const double* z, const double* E);
+/* EPH_FDM::solve, eph_fdm.h:267-400; *substeps receives the sub-step count */
+int eph_grid_solve(eph_grid* g, int* substeps);
+/* which: 0 T, 1 dT (deposit), 2 C_e, 3 rho_e, 4 kappa, 5 S */
+int eph_grid_get(eph_grid* g, int which, double* out);
+int eph_grid_set(eph_grid* g, int which, const double* in);
+/* EPH_FDM::get_index / get_T for n points (eph_fdm.h:182-187, 494-509) */
+int eph_grid_lookup(eph_grid* g, int n, const double* x, const double* y, const double* z,
+                    int* index, double* T);
+int eph_grid_info(const eph_grid* g, int* dims, double* box, double* last_r);
+int eph_grid_T_mean(eph_grid* g, double* out);        /* eph_fdm.h:189-196 */
+int eph_grid_save_state(eph_grid* g, const char* path);                  /* :226-265 */
+int eph_grid_save_temperature(eph_grid* g, const char* base, int n);     /* :198-224 */
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EPH_AMD_H */

@@ -1,0 +1,195 @@
+"""CPU checks of the product library's host side (no GPU needed).
+
+The library must load and export every symbol the headers declare; its own
+table builder must reproduce the reference's tables bit for bit; argument and
+file errors must be reported the way FixEPH reports them; and the harness
+that stands in for LAMMPS must produce exact ghosts, complete lists and
+correct pair forces.
+"""
+import gzip
+import hashlib
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+import eph_amd as ea
+from eph_amd import inputs
+import minisys
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+G = os.path.join(ROOT, "tests", "golden")
+HEADERS = [os.path.join(ROOT, "include", "eph_amd.h"), os.path.join(ROOT, "include", "eph_md.h")]
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a, dtype=np.float64).tobytes()).hexdigest()
+
+
+def test_library_exports_every_declared_symbol():
+    L = ea.lib()
+    names = ea.exported_symbols(HEADERS)
+    assert len(names) > 50
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+
+
+def test_product_spline_builder_matches_reference_goldens():
+    d = np.load(os.path.join(G, "spline.npz"))
+    assert np.array_equal(ea.spline_build(float(d["sin_dx"]), d["sin_y"]), d["sin_coeffs"])
+    for k in range(4):
+        assert np.array_equal(ea.spline_build(float(d[f"rand{k}_dx"]), d[f"rand{k}_y"]),
+                              d[f"rand{k}_coeffs"])
+
+
+def _beta_file(tmp_path, name):
+    if name == "beta_synth1.npz":
+        p = str(tmp_path / "s1.beta"); inputs.workload_beta(p, 1); return p
+    if name == "beta_synth4.npz":
+        p = str(tmp_path / "s4.beta"); inputs.workload_beta(p, 4); return p
+    src = os.path.join(G, name.replace("beta_", "").replace(".npz", ".beta.gz"))
+    p = str(tmp_path / os.path.basename(src)[:-3])
+    with gzip.open(src, "rb") as fi, open(p, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    return p
+
+
+@pytest.mark.parametrize("name", ["beta_synth1.npz", "beta_synth4.npz", "beta_Ni.npz",
+                                  "beta_Ni_model_4.npz"])
+def test_product_beta_tables_match_reference_goldens(tmp_path, name):
+    d = np.load(os.path.join(G, name))
+    path = _beta_file(tmp_path, name)
+    for e in range(int(d["n_elements"])):
+        for w in range(4):
+            rows, inv, rc, rho = ea.beta_table(path, e, w)
+            key = f"e{e}_t{w}"
+            assert inv == float(d[key + "_inv_dx"])
+            assert rc == float(d["r_cutoff"]) and rho == float(d["rho_cutoff"])
+            assert sha(rows) == str(d[key + "_sha256"]), key
+
+
+def _args(tmp_path, **kw):
+    p = str(tmp_path / "s1.beta")
+    inputs.workload_beta(p, 1)
+    return inputs.fix_args(p, ["Cu"], grid=(4, 4, 4), **kw)
+
+
+@pytest.mark.parametrize("mutate,code", [
+    (lambda a: a[:17], -1),                                   # too few arguments
+    (lambda a: a[:16] + ["/nonexistent.beta", "Cu"], -2),     # missing .beta
+    (lambda a: a[:17] + ["Xx"], -1),                          # element not in file
+    (lambda a: a[:10] + ["0"] + a[11:], -1),                  # non-positive grid
+    (lambda a: a[:5] + ["2"] + a[6:], -8),                    # PRB model: not built
+    (lambda a: a[:5] + ["7"] + a[6:], -1),                    # invalid model
+    (lambda a: a[:13] + ["/nonexistent.T"] + a[14:], -2),     # missing grid file
+])
+def test_fix_command_errors_like_the_reference(tmp_path, mutate, code):
+    args = mutate(_args(tmp_path))
+    with pytest.raises(ea.EphError) as e:
+        ea.FixEPH(args, 1, [0, 0, 0], [10, 10, 10], 1e-3, inputs.BOLTZ, inputs.FTM2V)
+    assert e.value.code == code
+
+
+def test_valid_fix_needs_a_device_and_says_so(tmp_path):
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a GPU is present")
+    except ImportError:
+        pass
+    with pytest.raises(ea.EphError) as e:
+        ea.FixEPH(_args(tmp_path), 1, [0, 0, 0], [10, 10, 10], 1e-3, inputs.BOLTZ, inputs.FTM2V)
+    assert e.value.code == -9
+
+
+def test_grid_file_errors(tmp_path):
+    p = str(tmp_path / "bad.T")
+    with open(p, "w") as f:
+        f.write("no comment\n")
+    with pytest.raises(ea.EphError) as e:
+        ea.Grid.load(p)
+    assert e.value.code == -3
+    # a temperature-dependent cell without a parameter file is rejected
+    q = str(tmp_path / "t.T")
+    inputs.write_grid(q, 2, 2, 2, (0, 1, 0, 1, 0, 1), 300.0, tdyn=1)
+    with pytest.raises(ea.EphError) as e:
+        ea.Grid.load(q)
+    assert e.value.code == -3
+
+
+def test_setmask_names_the_four_hooks():
+    assert ea.lib().eph_setmask() == 0b1111
+
+
+def test_harness_ghosts_lists_and_lattice():
+    md = ea.MD(4, seed=3, T=300.0, disp_sigma=0.05)
+    s = md.snapshot()
+    nl = s.nlocal
+    assert nl == 256
+    # ghost images are owner + shift exactly, shifts are multiples of L
+    assert np.array_equal(s.x[nl:], s.x[s.ghost_owner] + s.ghost_shift)
+    L = s.boxhi - s.boxlo
+    assert np.all(np.isin(np.round(s.ghost_shift / L), [-1, 0, 1]))
+    # every image within the ghost cutoff of the box exists exactly once
+    cut = 7.5
+    want = 0
+    for sx in (-1, 0, 1):
+        for sy in (-1, 0, 1):
+            for sz in (-1, 0, 1):
+                if (sx, sy, sz) == (0, 0, 0):
+                    continue
+                p = s.x[:nl] + np.array([sx, sy, sz]) * L
+                want += np.sum(np.all((p >= s.boxlo - cut) & (p < s.boxhi + cut), axis=1))
+    assert s.nghost == want
+    # fix list = all neighbours within r_cut + skin (brute force)
+    c2 = 6.5 ** 2
+    for i in range(0, nl, 7):
+        d = s.x - s.x[i]
+        js = set(np.nonzero((d * d).sum(1) < c2)[0].tolist()) - {i}
+        got = s.nbr_idx[s.nbr_off[i]:s.nbr_off[i + 1]].tolist()
+        assert len(got) == len(js) and set(got) == js
+    # zero momentum, temperature 300 K with 3N - 3 dof
+    v = s.v[:nl]
+    assert np.abs(v.sum(0)).max() < 1e-9
+    T = (v * v).sum() * inputs.MASS_CU * inputs.MVV2E / ((3 * nl - 3) * inputs.BOLTZ)
+    assert abs(T - 300.0) < 1e-9
+
+
+def test_harness_lj_forces_match_brute_force():
+    md = ea.MD(3, seed=2, T=300.0, disp_sigma=0.1)
+    s = md.snapshot()
+    md.lj(s)
+    eps, sig, rc = 0.4, 2.3, 5.5
+    for i in range(0, s.nlocal, 5):
+        d = s.x[i] - s.x
+        r2 = (d * d).sum(1)
+        m = (r2 < rc * rc) & (r2 > 0)
+        sr6 = (sig * sig / r2[m]) ** 3
+        fp = 24 * eps * (2 * sr6 * sr6 - sr6) / r2[m]
+        f = (d[m] * fp[:, None]).sum(0)
+        assert np.allclose(s.f[i], f, rtol=1e-12, atol=1e-12)
+    # Newton's third law over the periodic system
+    assert np.abs(s.f[:s.nlocal].sum(0)).max() < 1e-10
+
+
+def test_harness_pka_and_species():
+    md = ea.MD(6, ntypes=4, seed=9, T=300.0, pka_eV=1000.0)
+    s = md.snapshot()
+    nl = s.nlocal
+    counts = np.bincount(s.type[:nl])[1:]
+    assert list(counts) == [nl // 4] * 4
+    ke = 0.5 * inputs.MASS_CU * inputs.MVV2E * (s.v[:nl] ** 2).sum(1)
+    assert abs(ke.max() - 1000.0) < 1e-9
+
+
+def test_neighbour_rebuild_after_large_moves():
+    md = ea.MD(4, seed=4, T=300.0)
+    s = md.snapshot()
+    assert not md.needs_rebuild(s)
+    s.x[5, 0] += 1.01
+    assert md.needs_rebuild(s)
+    s.x[7, 1] -= 20.0  # leaves the box: the rebuild remaps it
+    md.rebuild(s)
+    s2 = md.snapshot()
+    assert np.all(s2.x[:s2.nlocal] >= s2.boxlo) and np.all(s2.x[:s2.nlocal] < s2.boxhi)

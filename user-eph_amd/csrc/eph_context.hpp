@@ -1,0 +1,154 @@
+// eph_context.hpp -- internal definition of the opaque handles of eph_amd.h.
+#pragma once
+
+#include <string>
+#include <vector>
+
+#include "eph_device.hpp"
+#include "eph_host.hpp"
+
+extern "C" {
+#include "eph_amd.h"
+}
+
+namespace eph {
+
+// Owning device allocation, grown on demand (never shrunk).
+template <typename T>
+struct DevArray {
+  T* p = nullptr;
+  size_t n = 0;
+  DevArray() = default;
+  DevArray(const DevArray&) = delete;
+  DevArray& operator=(const DevArray&) = delete;
+  ~DevArray() { release(); }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+  // ensure room for `want` elements; contents are not preserved
+  hipError_t reserve(size_t want) {
+    if (want <= n && p) return hipSuccess;
+    release();
+    size_t m = want < 64 ? 64 : want;
+    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
+    if (e != hipSuccess) {
+      p = nullptr;
+      return e;
+    }
+    n = m;
+    return hipMemset(p, 0, m * sizeof(T));
+  }
+};
+
+// Scratch of the ordered deposit (per item and per cell arrays).
+struct DepositBuffers {
+  DevArray<int> cell, count, start, fill, items, scan_tmp, qbuf;
+  DevArray<double> v1, v2, partial;
+  hipError_t reserve(size_t n_items, size_t G, size_t nblocks);
+  DepositView view(bool with_v2) {
+    DepositView d;
+    d.cell = cell.p; d.count = count.p; d.start = start.p; d.fill = fill.p; d.items = items.p;
+    d.v1 = v1.p; d.v2 = with_v2 ? v2.p : nullptr; d.partial = partial.p; d.scan_tmp = scan_tmp.p;
+    return d;
+  }
+};
+
+// Runs the error-returning HIP call and turns a failure into a status.
+#define EPH_HIP(obj, call)                                                        \
+  do {                                                                            \
+    hipError_t e_ = (call);                                                       \
+    if (e_ != hipSuccess) return (obj)->fail_hip(e_, #call);                      \
+  } while (0)
+
+}  // namespace eph
+
+struct eph_grid {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  bool own_stream = false;
+  std::string err;
+  eph::GridHost h;
+  eph::GridView v;
+  eph::DevArray<double> T, Tn, dT, S, rho, C, kappa, sum_tmp, minmax, E_y, E_dy, pts;
+  eph::DevArray<int8_t> flag, tdyn;
+  eph::DevArray<double4> C_T, kappa_T;
+  eph::DevArray<int> nonfinite, lookup_idx;
+  eph::DepositBuffers dep;
+  bool any_tdyn = false;
+  bool minmax_valid = false;
+  double cmin = 0, rmin = 0, kmax = 0;
+  double last_r = 0;
+  int last_steps = 0;
+
+  int fail(int code, const std::string& m) { err = m; return code; }
+  int fail_hip(hipError_t e, const char* what) {
+    err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    return e == hipErrorOutOfMemory ? EPH_ERR_OOM : EPH_ERR_HIP;
+  }
+  int upload();            // host description -> device (all arrays)
+  void refresh_minmax();   // host min/max of the stability estimate
+  int solve(int* substeps);
+  int insert_points(int n, const double* x, const double* y, const double* z, const double* E);
+  int download(int which, std::vector<double>& out);
+  int sync_params_from_device();  // C, kappa after temperature-dependent updates
+};
+
+struct eph_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // fix parameters (fix_eph.cpp:61-241)
+  int seed = 0, flags = 0, model = 0, ntypes = 0, T_freq = 0, groupbit = 1;
+  std::string T_out, T_state;
+  double dt = 0, boltz = 0, ftm2v = 0, eta = 0, dtv = 0, dtf = 0;
+  std::vector<int> type_map;
+  std::vector<double> mass_host;
+  eph::BetaModel beta;
+  eph::TablesView tabs;
+  eph::DevArray<double4> t_rho_r2, t_alpha, t_beta;
+  eph_grid* grid = nullptr;
+  // atoms
+  int nlocal = 0, nghost = 0, stride = 0, fstride = 0, maxnbr = 0;
+  bool have_atoms = false, have_list = false, have_forces = false;
+  eph::DevArray<double4> pos, vel, rw, xa, ghost_shift, stage4;
+  eph::DevArray<double> f, fE, fR, xi_in, stage, mass, scal;  // scal: [0] E_local [1] Ee
+  eph::DevArray<int> type, mask, tag, ghost_owner, nbr, nnbr, act, nact, csr_off, csr_idx,
+      flags_dev;
+  std::vector<int> type_host;
+  int xi_mode_next = 0;  // 1 once eph_set_noise injected values
+  long long step = 0;
+  double* pinned = nullptr;  // small pinned host scratch
+  // optional per-kernel timing
+  bool timing = false;
+  struct TimedLaunch { const char* name; hipEvent_t a, b; };
+  std::vector<TimedLaunch> pending;
+  std::vector<hipEvent_t> event_pool;
+  struct KernelStat { std::string name; double ms = 0; long count = 0; };
+  std::vector<KernelStat> stats;
+
+  int fail(int code, const std::string& m) { err = m; return code; }
+  int fail_hip(hipError_t e, const char* what) {
+    err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
+    return e == hipErrorOutOfMemory ? EPH_ERR_OOM : EPH_ERR_HIP;
+  }
+  eph::AtomsView atoms_view();
+  eph::StepParams step_params();
+  eph::WorkView work_view();
+  int reserve_atoms(int nl, int ng);
+  int set_atoms(int nl, int ng, const double* x, const double* v, const int* ty, const int* mk,
+                const int* tg);
+  int set_ghost_map(int ng, const int* owner, const double* shift);
+  int set_neighbors(int nl, const int* off, const int* idx, int neighmask);
+  int dev_initial_integrate();
+  int dev_post_force();
+  int dev_final_integrate();
+  int dev_end_of_step(long long ntimestep);
+  int check_flags();
+  // timing helpers
+  hipEvent_t take_event();
+  void timed_begin(const char* name, hipEvent_t* a);
+  void timed_end(const char* name, hipEvent_t a);
+  int resolve_timing();
+};

@@ -1,0 +1,162 @@
+// eph_device.hpp -- device data layout and the kernel launch interface.
+//
+// Layout in HBM (FP64 throughout):
+//   pos[nall]  double4 (x, y, z, element index)   gathered per neighbour
+//   vel[nall]  double4 (vx, vy, vz, 0)             gathered in the w sweep
+//   rw[nall]   double4 (w_x, w_y, w_z, rho)        gathered in the force sweep
+//   xa[nall]   double4 (xi_x, xi_y, xi_z, alpha)   gathered in the force sweep
+//   f, f_EPH, f_RNG: structure of arrays [3][nlocal] (streamed, not gathered)
+//   neighbour lists: transposed ELL, entry k of atom i at [k * stride + i],
+//     so lane i of a wavefront reads consecutive words (one thread per atom,
+//     neighbours summed in list order exactly as the reference loops do).
+//   tables: per element, knots of (a, b, c, d) as double4.
+//   T_e grid: flat double arrays in i + j nx + k nx ny order (eph_fdm.h:110).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace eph {
+
+struct TablesView {
+  int n_elem = 0;
+  int n_rho = 0;
+  double inv_dr2 = 0;         // 1 / d(r^2)
+  const double4* rho_r2 = nullptr;  // [n_elem][n_rho]
+  int n_beta = 0;
+  double inv_drho = 0;
+  const double4* alpha = nullptr;   // [n_elem][n_beta]
+  const double4* beta = nullptr;    // [n_elem][n_beta]
+  double rc2 = 0;
+  double rho_cut = 0;
+};
+
+struct GridView {
+  int nx = 1, ny = 1, nz = 1;
+  double x0 = 0, y0 = 0, z0 = 0;
+  double dx = 1, dy = 1, dz = 1;
+  double dV = 1;
+  double* T = nullptr;      // current T_e
+  double* Tn = nullptr;     // ping-pong buffer
+  double* dT = nullptr;     // deposited power density (dT_e)
+  const double* S = nullptr;
+  const double* rho = nullptr;
+  double* C = nullptr;
+  double* kappa = nullptr;
+  const int8_t* flag = nullptr;
+  const int8_t* tdyn = nullptr;
+  // temperature-dependent parameters
+  const double4* C_T = nullptr; int n_CT = 0; double inv_dT_C = 0;
+  const double4* kappa_T = nullptr; int n_kT = 0; double inv_dT_k = 0;
+  const double* E_y = nullptr; const double* E_dy = nullptr; int n_E = 0; double E_dx = 1;
+};
+
+struct AtomsView {
+  int nlocal = 0, nghost = 0;
+  int stride = 0;             // ELL column stride (>= nlocal, multiple of 64)
+  double4* pos = nullptr;
+  double4* vel = nullptr;
+  double* f = nullptr;        // [3][fstride]
+  int fstride = 0;
+  const int* type = nullptr;  // 1-based LAMMPS type
+  const int* mask = nullptr;
+  const int* tag = nullptr;
+  int groupbit = 1;
+};
+
+struct StepParams {
+  int flags = 0;
+  double eta = 0;        // sqrt(2 k_B / dt)
+  double dtv = 0, dtf = 0;
+  double dt = 0;
+  const double* mass = nullptr;  // [ntypes + 1]
+  uint32_t seed = 0;
+  uint64_t step = 0;
+  int xi_mode = 0;       // 0: zero, 1: injected [nlocal][3], 2: Philox stream
+  const double* xi_in = nullptr;
+};
+
+// Work buffers of one post_force / end_of_step
+struct WorkView {
+  const int* nbr = nullptr;   // full list, ELL
+  const int* nnbr = nullptr;
+  int* act = nullptr;         // active (r^2 < rc^2) entries, ELL, written by the density sweep
+  int* nact = nullptr;
+  double4* rw = nullptr;
+  double4* xa = nullptr;
+  double* fE = nullptr;       // [3][fstride]
+  double* fR = nullptr;       // [3][fstride]
+  int* flags_out = nullptr;   // bit 0: rho > rho_cutoff seen
+  const int* ghost_owner = nullptr;
+  const double4* ghost_shift = nullptr;
+};
+
+// Ordered deposit of per-atom values into grid cells (bit-identical to a
+// serial loop over atoms): cell of each item, counts, offsets and the
+// per-cell item lists.
+struct DepositView {
+  int* cell = nullptr;        // [n]
+  int* count = nullptr;       // [G + 1]
+  int* start = nullptr;       // [G + 1]
+  int* fill = nullptr;        // [G]
+  int* items = nullptr;       // [n]
+  double* v1 = nullptr;       // first value per item (friction part)
+  double* v2 = nullptr;       // second value per item (random part), may be null
+  double* partial = nullptr;  // [2 * nblocks] energy partial sums
+  int* scan_tmp = nullptr;    // block sums for the scan
+  int* maxcount = nullptr;    // [1]
+};
+
+constexpr int kBlock = 256;
+constexpr int kCellConstant = 0, kCellDynamic = 1, kCellZeroDerivative = 2;
+constexpr int kMaxSortInCell = 64;
+
+// ---- launch wrappers (eph_kernels.hip) ----
+hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);
+hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
+                             int* nn, int stride, hipStream_t s);
+hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
+                            const WorkView& w, hipStream_t s);
+hipError_t launch_ghost_rw_xa(const AtomsView& a, const WorkView& w, hipStream_t s);
+hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s);
+hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s);
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
+hipError_t launch_fill_double(double* p, long n, double v, hipStream_t s);
+// deposit of the fix (energy of friction / random forces at end of step)
+hipError_t launch_deposit_atoms(const AtomsView& a, const GridView& g, const StepParams& p,
+                                const WorkView& w, const DepositView& d, int nblocks, hipStream_t s);
+hipError_t launch_deposit_points(int n, const double* x, const double* y, const double* z,
+                                 const double* E, double dt, const GridView& g, const DepositView& d,
+                                 hipStream_t s);
+// qbuf: 2 + 2 G ints of queue space for cells holding more than
+// kMaxSortInCell items.  d.count must hold the per-cell counts.
+hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_items, int* qbuf,
+                                hipStream_t s);
+hipError_t launch_energy_reduce(const DepositView& d, int nblocks, double* E_local, double* Ee,
+                                hipStream_t s);
+// grid
+hipError_t launch_grid_tdyn_update(const GridView& g, hipStream_t s);
+hipError_t launch_grid_minmax(const GridView& g, double* out3, double* scratch, hipStream_t s);
+hipError_t launch_fdm_substep(const GridView& g, const double* Tin, double* Tout, double inner_dt,
+                              bool any_tdyn, hipStream_t s);
+hipError_t launch_grid_lookup(const GridView& g, int n, const double* x, const double* y,
+                              const double* z, int* idx, double* T, hipStream_t s);
+hipError_t launch_sum(const double* v, int n, double* out, double* scratch, hipStream_t s);
+hipError_t launch_nonfinite(const double* v, int n, int* flag, hipStream_t s);
+hipError_t launch_peratom(const AtomsView& a, const TablesView& t, const WorkView& w,
+                          double* out, hipStream_t s);
+// host-layout staging
+hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s);
+hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s);
+hipError_t launch_rows_into_soa(const double* rows, int n, double* soa, int stride, hipStream_t s);
+hipError_t launch_soa_into_rows(const double* soa, int n, int stride, double* rows, hipStream_t s);
+hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s);
+// harness
+hipError_t launch_lj(const AtomsView& a, const int* nbr, const int* nn, int stride, double eps,
+                     double sigma, double rc2, hipStream_t s);
+hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* partial, int nblocks,
+                            hipStream_t s);
+
+}  // namespace eph

@@ -1,0 +1,1022 @@
+// eph_kernels.hip -- gfx950 kernels of the fix eph hot path.
+//
+// Arithmetic contract: every kernel evaluates the reference's FP64
+// expressions in the reference's operation order, one thread per atom (or
+// per grid cell) summing its neighbours in list order, with FMA contraction
+// disabled at build time.  Results are therefore bit-identical to the serial
+// CPU FixEPH on the same inputs; no atomics touch floating-point sums.
+#include "eph_device.hpp"
+#include "eph_rng.hpp"
+
+namespace eph {
+
+namespace {
+
+// Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b % 8).  Remap so
+// that each XCD owns one contiguous range of atoms: neighbour gathers of a
+// range then stay in that XCD's L2.
+__device__ __forceinline__ int xcd_block() {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int x = b % kXcd, local = b / kXcd;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+__device__ __forceinline__ double cubic_at(const double4* __restrict__ tab, int n, double inv,
+                                           double x) {
+  int i = static_cast<int>(x * inv);
+  i = i < 0 ? 0 : (i < n - 1 ? i : n - 1);
+  const double4 c = tab[i];
+  return c.x + x * (c.y + x * (c.z + x * c.w));
+}
+
+// EPH_FDM::get_index (eph_fdm.h:494-509): floor, then periodic fold
+__device__ __forceinline__ int cell_of(const GridView& g, double x, double y, double z) {
+  int lx = static_cast<int>(floor((x - g.x0) / g.dx));
+  int px = static_cast<int>(floor(static_cast<double>(lx) / g.nx));
+  lx -= px * g.nx;
+  int ly = static_cast<int>(floor((y - g.y0) / g.dy));
+  int py = static_cast<int>(floor(static_cast<double>(ly) / g.ny));
+  ly -= py * g.ny;
+  int lz = static_cast<int>(floor((z - g.z0) / g.dz));
+  int pz = static_cast<int>(floor(static_cast<double>(lz) / g.nz));
+  lz -= pz * g.nz;
+  const int c = lx + ly * g.nx + lz * g.nx * g.ny;
+  // finite coordinates always land in range; non-finite ones (a run that has
+  // blown up) must not index outside the grid
+  return (c >= 0 && c < g.nx * g.ny * g.nz) ? c : 0;
+}
+
+template <typename T>
+__device__ __forceinline__ T block_sum(T v, T* sh) {
+  // fixed-order tree: deterministic for a given block size
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- comm
+__global__ __launch_bounds__(kBlock) void k_ghost_xv(AtomsView a, const int* __restrict__ owner,
+                                                     const double4* __restrict__ shift) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= a.nghost) return;
+  const int o = owner[g];
+  const double4 p = a.pos[o];
+  const double4 s = shift[g];
+  a.pos[a.nlocal + g] = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, p.w);
+  a.vel[a.nlocal + g] = a.vel[o];
+}
+
+__global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __restrict__ off,
+                                                       const int* __restrict__ idx, int neighmask,
+                                                       int* __restrict__ ell, int* __restrict__ nn,
+                                                       int stride) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= nlocal) return;
+  const int b = off[i], e = off[i + 1];
+  for (int k = b; k < e; ++k) ell[(k - b) * stride + i] = idx[k] & neighmask;
+  nn[i] = e - b;
+}
+
+// ------------------------------------------------- density + w (stage 1)
+// FixEPH::calculate_environment (fix_eph.cpp:431-466) followed by stage 1
+// of force_prl (fix_eph.cpp:702-741) and the noise draw of
+// fix_eph.cpp:854-862.  The density sweep also compacts the entries with
+// r^2 < rc^2 into an active list, which the w sweep and the force sweep
+// re-read in the same order (they skip exactly the other entries).
+template <int XI, bool FRIC, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
+                                                      WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  const int stride = a.stride;
+  const double4 pi = a.pos[i];
+  const bool grp = (a.mask[i] & a.groupbit) != 0;
+  double rho = 0.0;
+  int na = 0;
+  if (grp) {
+    const int nn = w.nnbr[i];
+    for (int k = 0; k < nn; ++k) {
+      const int j = w.nbr[k * stride + i];
+      const double4 pj = a.pos[j];
+      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      const double r2 = ex * ex + ey * ey + ez * ez;
+      if (r2 < t.rc2) {
+        const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+        rho += cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        w.act[na * stride + i] = j;
+        ++na;
+      }
+    }
+  }
+  w.nact[i] = na;
+  double alpha = 0.0;
+  if (grp && rho > 0.0) {
+    if (rho > t.rho_cut) {
+      atomicOr(w.flags_out, 1);  // eph_beta.h:189-194: alpha = 0 with a warning
+    } else {
+      const int ei = MULTI ? static_cast<int>(pi.w) : 0;
+      alpha = cubic_at(t.alpha + ei * t.n_beta, t.n_beta, t.inv_drho, rho);
+    }
+  }
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  if (FRIC && grp && rho > 0.0) {
+    const double4 vi = a.vel[i];
+    for (int k = 0; k < na; ++k) {
+      const int j = w.act[k * stride + i];
+      const double4 pj = a.pos[j];
+      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      const double r2 = ex * ex + ey * ey + ez * ez;
+      const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+      const double vr = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+      const double pre = alpha * vr / (rho * r2);
+      const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
+      const double var1 = pre * ev1;
+      const double4 vj = a.vel[j];
+      const double ev2 = ex * vj.x + ey * vj.y + ez * vj.z;
+      const double var2 = pre * ev2;
+      const double dvar = var1 - var2;
+      wx += dvar * ex;
+      wy += dvar * ey;
+      wz += dvar * ez;
+    }
+  }
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  if (XI == 1 && grp) {
+    x0 = p.xi_in[3 * i + 0];
+    x1 = p.xi_in[3 * i + 1];
+    x2 = p.xi_in[3 * i + 2];
+  } else if (XI == 2 && grp) {
+    double g3[3];
+    gaussian3(p.seed, static_cast<uint32_t>(a.tag[i]), p.step, g3);
+    x0 = g3[0]; x1 = g3[1]; x2 = g3[2];
+  }
+  w.rw[i] = make_double4(wx, wy, wz, rho);
+  w.xa[i] = make_double4(x0, x1, x2, alpha);
+}
+
+// forward communication of rho, w (fix_eph.cpp:743-744, 870-871) and xi
+// (:863-864) on one process: ghost <- owner
+__global__ __launch_bounds__(kBlock) void k_ghost_rw_xa(AtomsView a, WorkView w) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= a.nghost) return;
+  const int o = w.ghost_owner[g];
+  w.rw[a.nlocal + g] = w.rw[o];
+  w.xa[a.nlocal + g] = w.xa[o];
+}
+
+// ------------------------------------------- force_prl stages 2 and 3 fused
+// fix_eph.cpp:748-787 (friction) and :791-836 (random), then the force
+// application of :892-906.  Both sums share the neighbour gather; each
+// accumulates in list order exactly as its reference loop does.
+template <bool FRIC, bool RAND, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
+                                                       StepParams p, WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  const int stride = a.stride;
+  const bool grp = (a.mask[i] & a.groupbit) != 0;
+  const double4 rwi = w.rw[i];
+  const double rho_i = rwi.w;
+  double fex = 0.0, fey = 0.0, fez = 0.0, frx = 0.0, fry = 0.0, frz = 0.0;
+  if (grp && rho_i > 0.0) {
+    const double4 pi = a.pos[i];
+    const double4 xai = w.xa[i];
+    const double alpha_i = xai.w;
+    const int ei = MULTI ? static_cast<int>(pi.w) : 0;
+    const int na = w.nact[i];
+    for (int k = 0; k < na; ++k) {
+      const int j = w.act[k * stride + i];
+      const double4 rwj = w.rw[j];
+      const double rho_j = rwj.w;
+      if (!(rho_j > 0.0)) continue;
+      const double4 pj = a.pos[j];
+      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      const double r2 = ex * ex + ey * ey + ez * ez;
+      const double4 xaj = w.xa[j];
+      const double alpha_j = xaj.w;
+      const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+      const double vr_ji = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+      const double vr_ij = MULTI ? cubic_at(t.rho_r2 + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
+      if (FRIC) {
+        const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
+        const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
+        const double ev2 = ex * rwj.x + ey * rwj.y + ez * rwj.z;
+        const double var2 = alpha_j * vr_ij * ev2 / (rho_j * r2);
+        const double dvar = var1 - var2;
+        fex -= dvar * ex;
+        fey -= dvar * ey;
+        fez -= dvar * ez;
+      }
+      if (RAND) {
+        const double ex1 = ex * xai.x + ey * xai.y + ez * xai.z;
+        const double var1 = alpha_i * vr_ji * ex1 / (rho_i * r2);
+        const double ex2 = ex * xaj.x + ey * xaj.y + ez * xaj.z;
+        const double var2 = alpha_j * vr_ij * ex2 / (rho_j * r2);
+        const double dvar = var1 - var2;
+        frx += dvar * ex;
+        fry += dvar * ey;
+        frz += dvar * ez;
+      }
+    }
+    if (RAND) {
+      const double Te = g.T[cell_of(g, pi.x, pi.y, pi.z)];
+      const double var = p.eta * sqrt(Te);
+      frx *= var;
+      fry *= var;
+      frz *= var;
+    }
+  }
+  const int fs = a.fstride;
+  if (FRIC) {
+    w.fE[i] = fex; w.fE[fs + i] = fey; w.fE[2 * fs + i] = fez;
+  }
+  if (RAND) {
+    w.fR[i] = frx; w.fR[fs + i] = fry; w.fR[2 * fs + i] = frz;
+  }
+  const bool addE = FRIC && !(p.flags & 0x10);
+  const bool addR = RAND && !(p.flags & 0x20);
+  if (addE || addR) {
+    double fx = a.f[i], fy = a.f[fs + i], fz = a.f[2 * fs + i];
+    if (addE) { fx = fx + fex; fy = fy + fey; fz = fz + fez; }
+    if (addR) { fx = fx + frx; fy = fy + fry; fz = fz + frz; }
+    a.f[i] = fx; a.f[fs + i] = fy; a.f[2 * fs + i] = fz;
+  }
+}
+
+// --------------------------------------------------------- integrator
+// FixEPH::initial_integrate / final_integrate (fix_eph.cpp:305-348)
+template <bool INITIAL>
+__global__ __launch_bounds__(kBlock) void k_integrate(AtomsView a, StepParams p) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  if (!(a.mask[i] & a.groupbit)) return;
+  const int fs = a.fstride;
+  const double dtfm = p.dtf / p.mass[a.type[i]];
+  double4 v = a.vel[i];
+  v.x += dtfm * a.f[i];
+  v.y += dtfm * a.f[fs + i];
+  v.z += dtfm * a.f[2 * fs + i];
+  a.vel[i] = v;
+  if (INITIAL) {
+    double4 x = a.pos[i];
+    x.x += p.dtv * v.x;
+    x.y += p.dtv * v.y;
+    x.z += p.dtv * v.z;
+    a.pos[i] = x;
+  }
+}
+
+__global__ void k_fill_int(int* p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_double(double* p, long n, double v) {
+  const long i = static_cast<long>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
+}
+
+// ---------------------------------------------------------- deposit
+// FixEPH::end_of_step (fix_eph.cpp:364-390): per-atom energies of the
+// friction and random forces at end-of-step v, their cell, and block sums
+// of E_local.
+template <bool FRIC, bool RAND>
+__global__ __launch_bounds__(kBlock) void k_deposit_atoms(AtomsView a, GridView g, StepParams p,
+                                                          WorkView w, DepositView d) {
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  double eF = 0.0, eR = 0.0;
+  if (i < a.nlocal) {
+    int c = -1;
+    double v1 = 0.0, v2 = 0.0;
+    if (a.mask[i] & a.groupbit) {
+      const double4 x = a.pos[i];
+      const double4 v = a.vel[i];
+      c = cell_of(g, x.x, x.y, x.z);
+      const int fs = a.fstride;
+      const double prescale = g.dV * p.dt;
+      if (FRIC) {
+        double dE = 0.0;
+        dE -= w.fE[i] * v.x * p.dt;
+        dE -= w.fE[fs + i] * v.y * p.dt;
+        dE -= w.fE[2 * fs + i] * v.z * p.dt;
+        eF = dE;
+        v1 = dE / prescale;
+      }
+      if (RAND) {
+        double dE = 0.0;
+        dE -= w.fR[i] * v.x * p.dt;
+        dE -= w.fR[fs + i] * v.y * p.dt;
+        dE -= w.fR[2 * fs + i] * v.z * p.dt;
+        eR = dE;
+        v2 = dE / prescale;
+      }
+      atomicAdd(&d.count[c], 1);
+    }
+    d.cell[i] = c;
+    d.v1[i] = v1;
+    d.v2[i] = v2;
+  }
+  const double sF = block_sum(eF, sh);
+  const double sR = block_sum(eR, sh);
+  if (threadIdx.x == 0) {
+    d.partial[2 * blockIdx.x] = sF;
+    d.partial[2 * blockIdx.x + 1] = sR;
+  }
+}
+
+// EPH_FDM::insert_energy for a list of points (eph_fdm.h:172-179)
+__global__ __launch_bounds__(kBlock) void k_deposit_points(int n, const double* x, const double* y,
+                                                           const double* z, const double* E,
+                                                           double dt, GridView g, DepositView d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of(g, x[i], y[i], z[i]);
+  d.cell[i] = c;
+  d.v1[i] = E[i] / (g.dV * dt);
+  atomicAdd(&d.count[c], 1);
+}
+
+// exclusive scan of count[0..n) into start[0..n] (three phases)
+constexpr int kScanBlock = 1024;
+constexpr int kScanItems = 4;  // per thread
+__global__ __launch_bounds__(kScanBlock) void k_scan_reduce(const int* in, int n, int* block_sums) {
+  __shared__ int sh[kScanBlock];
+  const int base = blockIdx.x * kScanBlock * kScanItems;
+  int s = 0;
+  for (int k = 0; k < kScanItems; ++k) {
+    const int idx = base + k * kScanBlock + threadIdx.x;
+    if (idx < n) s += in[idx];
+  }
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) block_sums[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_blocks(int* block_sums, int nb) {
+  // single block: exclusive scan of nb block sums, chunked by kScanBlock
+  __shared__ int sh[kScanBlock];
+  int carry = 0;
+  for (int base = 0; base < nb; base += kScanBlock) {
+    const int idx = base + threadIdx.x;
+    const int v = idx < nb ? block_sums[idx] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      const int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    const int incl = sh[threadIdx.x];
+    const int total = sh[kScanBlock - 1];
+    if (idx < nb) block_sums[idx] = carry + incl - v;
+    __syncthreads();
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kScanBlock) void k_scan_apply(const int* in, int n, const int* block_off,
+                                                           int* out) {
+  __shared__ int sh[kScanBlock];
+  const int base = blockIdx.x * kScanBlock * kScanItems;
+  int carry = block_off[blockIdx.x];
+  for (int k = 0; k < kScanItems; ++k) {
+    const int idx = base + k * kScanBlock + threadIdx.x;
+    const int v = idx < n ? in[idx] : 0;
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int off = 1; off < kScanBlock; off <<= 1) {
+      const int add = threadIdx.x >= off ? sh[threadIdx.x - off] : 0;
+      __syncthreads();
+      sh[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (idx < n) out[idx] = carry + sh[threadIdx.x] - v;
+    if (idx == n - 1) out[n] = carry + sh[threadIdx.x];
+    const int total = sh[kScanBlock - 1];
+    __syncthreads();
+    carry += total;
+  }
+}
+
+__global__ __launch_bounds__(kBlock) void k_scatter(int n, DepositView d) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = d.cell[i];
+  if (c < 0) return;
+  const int slot = d.start[c] + atomicAdd(&d.fill[c], 1);
+  d.items[slot] = i;
+}
+
+// Per cell: dT_e[c] += v1 of its atoms in ascending atom order, then += v2 in
+// ascending order -- the serial order of the two loops of fix_eph.cpp:364-390.
+// Cells with up to kMaxSortInCell items sort them by insertion in place;
+// larger cells are queued for a workgroup (medium) or an ordered scan over
+// all items (huge).  d.count is reused as the queue: count[G] holds the
+// number of queued cells and the queue follows in fill[].
+__global__ __launch_bounds__(kBlock) void k_cell_sum_small(GridView g, DepositView d, int ncell,
+                                                           int* queue, int* qn) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  const int s = d.start[c], n = d.start[c + 1] - s;
+  if (n == 0) return;
+  if (n > kMaxSortInCell) {
+    const int q = atomicAdd(qn, 1);
+    queue[q] = c;
+    return;
+  }
+  int* it = d.items + s;
+  for (int k = 1; k < n; ++k) {
+    const int v = it[k];
+    int m = k - 1;
+    while (m >= 0 && it[m] > v) {
+      it[m + 1] = it[m];
+      --m;
+    }
+    it[m + 1] = v;
+  }
+  double acc = g.dT[c];
+  if (d.v1)
+    for (int k = 0; k < n; ++k) acc += d.v1[it[k]];
+  if (d.v2)
+    for (int k = 0; k < n; ++k) acc += d.v2[it[k]];
+  g.dT[c] = acc;
+}
+
+constexpr int kMedBlock = 1024;
+constexpr int kMedMax = 8192;
+// one workgroup per queued cell: bitonic sort of up to kMedMax items in LDS,
+// then the ordered sums; larger cells fall through to k_cell_sum_huge
+__global__ __launch_bounds__(kMedBlock) void k_cell_sum_medium(GridView g, DepositView d,
+                                                               const int* queue, const int* qn,
+                                                               int* huge, int* hn) {
+  __shared__ int key[kMedMax];
+  const int nq = *qn;
+  for (int q = blockIdx.x; q < nq; q += gridDim.x) {
+    const int c = queue[q];
+    const int s = d.start[c], n = d.start[c + 1] - s;
+    if (n > kMedMax) {
+      if (threadIdx.x == 0) huge[atomicAdd(hn, 1)] = c;
+      continue;
+    }
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int k = threadIdx.x; k < m; k += kMedBlock) key[k] = k < n ? d.items[s + k] : 0x7fffffff;
+    __syncthreads();
+    for (int size = 2; size <= m; size <<= 1) {
+      for (int stride = size >> 1; stride > 0; stride >>= 1) {
+        for (int k = threadIdx.x; k < m; k += kMedBlock) {
+          const int l = k ^ stride;
+          if (l > k) {
+            const bool up = (k & size) == 0;
+            const int a = key[k], b = key[l];
+            if ((a > b) == up) {
+              key[k] = b;
+              key[l] = a;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) {
+      double acc = g.dT[c];
+      if (d.v1)
+        for (int k = 0; k < n; ++k) acc += d.v1[key[k]];
+      if (d.v2)
+        for (int k = 0; k < n; ++k) acc += d.v2[key[k]];
+      g.dT[c] = acc;
+    }
+    __syncthreads();
+  }
+}
+
+// cells beyond kMedMax items: stream all items in ascending order; thread 0
+// adds the values of this cell one by one (serial order by construction)
+__global__ __launch_bounds__(kBlock) void k_cell_sum_huge(GridView g, DepositView d, int n_items,
+                                                          const int* huge, const int* hn) {
+  __shared__ double val[kBlock];
+  __shared__ int hit[kBlock];
+  const int nh = *hn;
+  for (int q = blockIdx.x; q < nh; q += gridDim.x) {
+    const int c = huge[q];
+    double acc = g.dT[c];  // meaningful in thread 0 only
+    for (int pass = 0; pass < 2; ++pass) {
+      const double* v = pass == 0 ? d.v1 : d.v2;
+      if (!v) continue;
+      for (int base = 0; base < n_items; base += kBlock) {
+        const int i = base + threadIdx.x;
+        const bool in = i < n_items && d.cell[i] == c;
+        hit[threadIdx.x] = in;
+        val[threadIdx.x] = in ? v[i] : 0.0;
+        __syncthreads();
+        if (threadIdx.x == 0)
+          for (int k = 0; k < kBlock; ++k)
+            if (hit[k]) acc += val[k];
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) g.dT[c] = acc;
+    __syncthreads();
+  }
+}
+
+// E_local = sum of friction then random energies; Ee += E_local
+__global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, int nb,
+                                                        double* E_local, double* Ee) {
+  __shared__ double sh[1024];
+  double sF = 0.0, sR = 0.0;
+  for (int b = threadIdx.x; b < nb; b += 1024) {
+    sF += partial[2 * b];
+    sR += partial[2 * b + 1];
+  }
+  sF = block_sum(sF, sh);
+  sR = block_sum(sR, sh);
+  if (threadIdx.x == 0) {
+    const double E = sF + sR;
+    *E_local = E;
+    *Ee += E;
+  }
+}
+
+// ------------------------------------------------------------- grid
+// C_e(T), kappa(T) of temperature-dependent cells (eph_fdm.h:280-287)
+__global__ __launch_bounds__(kBlock) void k_grid_tdyn_update(GridView g, int n) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= n || !g.tdyn[c]) return;
+  const double T = g.T[c];
+  g.C[c] = cubic_at(g.C_T, g.n_CT, g.inv_dT_C, T);
+  g.kappa[c] = cubic_at(g.kappa_T, g.n_kT, g.inv_dT_k, T);
+}
+
+// smallest C_e and rho_e, largest kappa over non-constant cells, seeded with
+// cell 0 (eph_fdm.h:289-300); min/max are exact, so the order is free
+__global__ __launch_bounds__(1024) void k_grid_minmax(GridView g, int n, double* out3) {
+  __shared__ double s0[1024], s1[1024], s2[1024];
+  double cmin = g.C[0], rmin = g.rho[0], kmax = g.kappa[0];
+  for (int c = 1 + threadIdx.x; c < n; c += 1024) {
+    if (g.flag[c] != 0) {
+      cmin = g.C[c] < cmin ? g.C[c] : cmin;
+      rmin = g.rho[c] < rmin ? g.rho[c] : rmin;
+      kmax = g.kappa[c] > kmax ? g.kappa[c] : kmax;
+    }
+  }
+  s0[threadIdx.x] = cmin; s1[threadIdx.x] = rmin; s2[threadIdx.x] = kmax;
+  __syncthreads();
+  for (int s = 512; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      s0[threadIdx.x] = s0[threadIdx.x + s] < s0[threadIdx.x] ? s0[threadIdx.x + s] : s0[threadIdx.x];
+      s1[threadIdx.x] = s1[threadIdx.x + s] < s1[threadIdx.x] ? s1[threadIdx.x + s] : s1[threadIdx.x];
+      s2[threadIdx.x] = s2[threadIdx.x + s] > s2[threadIdx.x] ? s2[threadIdx.x + s] : s2[threadIdx.x];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) { out3[0] = s0[0]; out3[1] = s1[0]; out3[2] = s2[0]; }
+}
+
+__device__ __forceinline__ double linear_fwd(const GridView& g, double x) {
+  // EPH_Linear::operator() (eph_linear.h:42-49); the top knot, whose slope
+  // the reference reads past its table, returns the knot value
+  if (!(x >= 0.0)) return 0.0;
+  const double q = x / g.E_dx;
+  if (q >= static_cast<double>(g.n_E)) return 0.0;
+  const int idx = static_cast<int>(q);
+  const double delta = x - idx * g.E_dx;
+  if (idx >= g.n_E - 1) return g.E_y[idx];
+  return g.E_y[idx] + g.E_dy[idx] * delta;
+}
+
+__device__ __forceinline__ double linear_rev(const GridView& g, double y) {
+  // EPH_Linear::reverse_lookup (eph_linear.h:52-61): upper_bound, step back
+  int lo = 0, hi = g.n_E;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (y < g.E_y[mid]) hi = mid; else lo = mid + 1;
+  }
+  if (lo >= g.n_E || lo == 0) return 0.0;
+  const int idx = lo - 1;
+  return idx * g.E_dx + 1. / g.E_dy[idx] * (y - g.E_y[idx]);
+}
+
+// one explicit sub-step of EPH_FDM::solve (eph_fdm.h:315-395): the 7-point
+// variable-kappa stencil on Tin, the update and the clamp, into Tout
+template <bool TDYN>
+__global__ __launch_bounds__(kBlock) void k_fdm_substep(GridView g, const double* __restrict__ Tin,
+                                                        double* __restrict__ Tout, double inner_dt) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  const int nx = g.nx, ny = g.ny, nz = g.nz;
+  if (r >= nx * ny * nz) return;
+  const int i = r % nx, j = (r / nx) % ny, k = r / (nx * ny);
+  const int fr = g.flag[r];
+  double ddT = 0.0;
+  if (fr != kCellZeroDerivative) {
+    const double Tr = Tin[r], kr = g.kappa[r];
+    int p, q;
+    // x
+    p = i > 0 ? r - 1 : r + (nx - 1);
+    q = i < nx - 1 ? r + 1 : r - (nx - 1);
+    if (g.flag[q] == kCellZeroDerivative) q = r;
+    else if (g.flag[p] == kCellZeroDerivative) p = r;
+    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dx / g.dx / 4.0;
+    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dx / g.dx);
+    // y
+    p = j > 0 ? r - nx : r + (ny - 1) * nx;
+    q = j < ny - 1 ? r + nx : r - (ny - 1) * nx;
+    if (g.flag[q] == kCellZeroDerivative) q = r;
+    else if (g.flag[p] == kCellZeroDerivative) p = r;
+    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dy / g.dy / 4.0;
+    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dy / g.dy);
+    // z
+    const int pl = nx * ny;
+    p = k > 0 ? r - pl : r + (nz - 1) * pl;
+    q = k < nz - 1 ? r + pl : r - (nz - 1) * pl;
+    if (g.flag[q] == kCellZeroDerivative) q = r;
+    else if (g.flag[p] == kCellZeroDerivative) p = r;
+    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dz / g.dz / 4.0;
+    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dz / g.dz);
+  }
+  double T = Tin[r];
+  if (fr == kCellDynamic) {
+    if (TDYN && g.tdyn[r] == 1) {
+      double E = linear_fwd(g, T);
+      E += (ddT + g.dT[r] + g.S[r]) / g.rho[r] * inner_dt;
+      T = linear_rev(g, E);
+    } else {
+      const double prescaler = g.rho[r] * g.C[r];
+      T += (ddT + g.dT[r] + g.S[r]) / prescaler * inner_dt;
+    }
+  }
+  if (T < 0.0) T = 0.0;
+  Tout[r] = T;
+}
+
+__global__ __launch_bounds__(kBlock) void k_grid_lookup(GridView g, int n, const double* x,
+                                                        const double* y, const double* z, int* idx,
+                                                        double* T) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const int c = cell_of(g, x[i], y[i], z[i]);
+  if (idx) idx[i] = c;
+  if (T) T[i] = g.T[c];
+}
+
+// deterministic two-level sum (block partials in fixed order)
+__global__ __launch_bounds__(kBlock) void k_sum_partial(const double* v, int n, double* partial) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < n; i += gridDim.x * kBlock) s += v[i];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_sum_final(const double* partial, int nb, double* out) {
+  __shared__ double sh[kBlock];
+  double s = 0.0;
+  for (int b = threadIdx.x; b < nb; b += kBlock) s += partial[b];
+  s = block_sum(s, sh);
+  if (threadIdx.x == 0) *out = s;
+}
+
+__global__ __launch_bounds__(kBlock) void k_nonfinite(const double* v, int n, int* flag) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n && !isfinite(v[i])) atomicOr(flag, 2);
+}
+
+// per-atom array of fix_eph.cpp:406-428
+__global__ __launch_bounds__(kBlock) void k_peratom(AtomsView a, TablesView t, WorkView w,
+                                                    double* out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  double* o = out + 8 * static_cast<long>(i);
+  if (a.mask[i] & a.groupbit) {
+    const double rho = w.rw[i].w;
+    const int e = static_cast<int>(a.pos[i].w);
+    o[0] = rho;
+    o[1] = rho > t.rho_cut ? 0.0 : cubic_at(t.beta + e * t.n_beta, t.n_beta, t.inv_drho, rho);
+    const int fs = a.fstride;
+    o[2] = w.fE ? w.fE[i] : 0.0;
+    o[3] = w.fE ? w.fE[fs + i] : 0.0;
+    o[4] = w.fE ? w.fE[2 * fs + i] : 0.0;
+    o[5] = w.fR ? w.fR[i] : 0.0;
+    o[6] = w.fR ? w.fR[fs + i] : 0.0;
+    o[7] = w.fR ? w.fR[2 * fs + i] : 0.0;
+  } else {
+    for (int k = 0; k < 8; ++k) o[k] = 0.0;
+  }
+}
+
+// ---------------------------------------------------------- harness
+// pair lj/cut, full list, Newton off: f_i = sum_j fpair (x_i - x_j)
+__global__ __launch_bounds__(kBlock) void k_lj(AtomsView a, const int* __restrict__ nbr,
+                                               const int* __restrict__ nn, int stride, double lj1,
+                                               double lj2, double rc2) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  const double4 pi = a.pos[i];
+  double fx = 0.0, fy = 0.0, fz = 0.0;
+  const int n = nn[i];
+  for (int k = 0; k < n; ++k) {
+    const int j = nbr[k * stride + i];
+    const double4 pj = a.pos[j];
+    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
+    const double r2 = dx * dx + dy * dy + dz * dz;
+    if (r2 < rc2) {
+      const double r2inv = 1.0 / r2;
+      const double r6inv = r2inv * r2inv * r2inv;
+      const double fpair = r6inv * (lj1 * r6inv - lj2) * r2inv;
+      fx += dx * fpair;
+      fy += dy * fpair;
+      fz += dz * fpair;
+    }
+  }
+  const int fs = a.fstride;
+  a.f[i] = fx;
+  a.f[fs + i] = fy;
+  a.f[2 * fs + i] = fz;
+}
+
+__global__ __launch_bounds__(kBlock) void k_max_disp2(AtomsView a, const double4* ref,
+                                                      double* partial) {
+  __shared__ double sh[kBlock];
+  double m = 0.0;
+  for (int i = blockIdx.x * kBlock + threadIdx.x; i < a.nlocal; i += gridDim.x * kBlock) {
+    const double4 p = a.pos[i], q = ref[i];
+    const double dx = p.x - q.x, dy = p.y - q.y, dz = p.z - q.z;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    m = d2 > m ? d2 : m;
+  }
+  sh[threadIdx.x] = m;
+  __syncthreads();
+  for (int s = kBlock / 2; s > 0; s >>= 1) {
+    if (threadIdx.x < s) sh[threadIdx.x] = sh[threadIdx.x + s] > sh[threadIdx.x] ? sh[threadIdx.x + s] : sh[threadIdx.x];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
+}
+
+// ---------------------------------------------------- host layout staging
+// LAMMPS [n][3] rows <-> device double4 (keeping .w) and [3][stride] arrays
+__global__ __launch_bounds__(kBlock) void k_rows_into_d4(const double* __restrict__ rows, int n,
+                                                         double4* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  double4 o = out[i];
+  o.x = rows[3 * i]; o.y = rows[3 * i + 1]; o.z = rows[3 * i + 2];
+  out[i] = o;
+}
+
+__global__ __launch_bounds__(kBlock) void k_d4_into_rows(const double4* __restrict__ in, int n,
+                                                         double* __restrict__ rows) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double4 o = in[i];
+  rows[3 * i] = o.x; rows[3 * i + 1] = o.y; rows[3 * i + 2] = o.z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_rows_into_soa(const double* __restrict__ rows, int n,
+                                                          double* __restrict__ soa, int stride) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  soa[i] = rows[3 * i]; soa[stride + i] = rows[3 * i + 1]; soa[2 * stride + i] = rows[3 * i + 2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_soa_into_rows(const double* __restrict__ soa, int n,
+                                                          int stride, double* __restrict__ rows) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  rows[3 * i] = soa[i]; rows[3 * i + 1] = soa[stride + i]; rows[3 * i + 2] = soa[2 * stride + i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_d4_component(const double4* __restrict__ in, int n,
+                                                         int comp, double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double4 o = in[i];
+  out[i] = comp == 0 ? o.x : comp == 1 ? o.y : comp == 2 ? o.z : o.w;
+}
+
+// ================================================================ wrappers
+static inline int nblk(long n, int b = kBlock) { return static_cast<int>((n + b - 1) / b); }
+
+hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s) {
+  if (a.nghost == 0) return hipSuccess;
+  k_ghost_xv<<<nblk(a.nghost), kBlock, 0, s>>>(a, owner, shift);
+  return hipGetLastError();
+}
+
+hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
+                             int* nn, int stride, hipStream_t s) {
+  if (nlocal == 0) return hipSuccess;
+  k_csr_to_ell<<<nblk(nlocal), kBlock, 0, s>>>(nlocal, off, idx, neighmask, ell, nn, stride);
+  return hipGetLastError();
+}
+
+hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
+                            const WorkView& w, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  const bool fric = p.flags & 0x01;
+  const bool multi = t.n_elem > 1;
+  const int g = nblk(a.nlocal);
+#define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, w)
+#define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
+#define EPH_DW_F(XI) if (fric) { EPH_DW_M(XI, true); } else { EPH_DW_M(XI, false); }
+  switch (p.xi_mode) {
+    case 1: EPH_DW_F(1); break;
+    case 2: EPH_DW_F(2); break;
+    default: EPH_DW_F(0); break;
+  }
+#undef EPH_DW_F
+#undef EPH_DW_M
+#undef EPH_DW
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_rw_xa(const AtomsView& a, const WorkView& w, hipStream_t s) {
+  if (a.nghost == 0) return hipSuccess;
+  k_ghost_rw_xa<<<nblk(a.nghost), kBlock, 0, s>>>(a, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
+  if (!fric && !rnd) return hipSuccess;
+  const int gr = nblk(a.nlocal);
+#define EPH_PF(F, R, M) k_prl_forces<F, R, M><<<gr, kBlock, 0, s>>>(a, t, g, p, w)
+#define EPH_PF_M(F, R) if (multi) EPH_PF(F, R, true); else EPH_PF(F, R, false)
+  if (fric && rnd) { EPH_PF_M(true, true); }
+  else if (fric) { EPH_PF_M(true, false); }
+  else { EPH_PF_M(false, true); }
+#undef EPH_PF_M
+#undef EPH_PF
+  return hipGetLastError();
+}
+
+hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  if (initial) k_integrate<true><<<nblk(a.nlocal), kBlock, 0, s>>>(a, p);
+  else k_integrate<false><<<nblk(a.nlocal), kBlock, 0, s>>>(a, p);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_fill_int<<<nblk(n), kBlock, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_fill_double(double* p, long n, double v, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  k_fill_double<<<nblk(n), kBlock, 0, s>>>(p, n, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit_atoms(const AtomsView& a, const GridView& g, const StepParams& p,
+                                const WorkView& w, const DepositView& d, int nblocks, hipStream_t s) {
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02;
+  if (nblocks == 0) return hipSuccess;
+  if (fric && rnd) k_deposit_atoms<true, true><<<nblocks, kBlock, 0, s>>>(a, g, p, w, d);
+  else if (fric) k_deposit_atoms<true, false><<<nblocks, kBlock, 0, s>>>(a, g, p, w, d);
+  else if (rnd) k_deposit_atoms<false, true><<<nblocks, kBlock, 0, s>>>(a, g, p, w, d);
+  else k_deposit_atoms<false, false><<<nblocks, kBlock, 0, s>>>(a, g, p, w, d);
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit_points(int n, const double* x, const double* y, const double* z,
+                                 const double* E, double dt, const GridView& g, const DepositView& d,
+                                 hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_deposit_points<<<nblk(n), kBlock, 0, s>>>(n, x, y, z, E, dt, g, d);
+  return hipGetLastError();
+}
+
+// count[] -> start[] -> items[] -> ordered per-cell sums into g.dT
+hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_items, int* qbuf,
+                                hipStream_t s) {
+  const int G = g.nx * g.ny * g.nz;
+  const int per = kScanBlock * kScanItems;
+  const int nb = (G + per - 1) / per;
+  k_scan_reduce<<<nb, kScanBlock, 0, s>>>(d.count, G, d.scan_tmp);
+  k_scan_blocks<<<1, kScanBlock, 0, s>>>(d.scan_tmp, nb);
+  k_scan_apply<<<nb, kScanBlock, 0, s>>>(d.count, G, d.scan_tmp, d.start);
+  // qbuf layout: [0] medium count, [1] huge count, [2 ..] medium queue,
+  // [2 + G ..] huge queue
+  int* qn = qbuf;
+  int* hn = qbuf + 1;
+  k_fill_int<<<1, 64, 0, s>>>(qbuf, 2, 0);
+  k_fill_int<<<nblk(G), kBlock, 0, s>>>(d.fill, G, 0);
+  if (n_items > 0) k_scatter<<<nblk(n_items), kBlock, 0, s>>>(n_items, d);
+  int* queue = qbuf + 2;
+  int* huge = qbuf + 2 + G;
+  k_cell_sum_small<<<nblk(G), kBlock, 0, s>>>(g, d, G, queue, qn);
+  k_cell_sum_medium<<<64, kMedBlock, 0, s>>>(g, d, queue, qn, huge, hn);
+  k_cell_sum_huge<<<16, kBlock, 0, s>>>(g, d, n_items, huge, hn);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_reduce(const DepositView& d, int nblocks, double* E_local, double* Ee,
+                                hipStream_t s) {
+  k_energy_reduce<<<1, 1024, 0, s>>>(d.partial, nblocks, E_local, Ee);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_tdyn_update(const GridView& g, hipStream_t s) {
+  const int n = g.nx * g.ny * g.nz;
+  k_grid_tdyn_update<<<nblk(n), kBlock, 0, s>>>(g, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_minmax(const GridView& g, double* out3, double*, hipStream_t s) {
+  k_grid_minmax<<<1, 1024, 0, s>>>(g, g.nx * g.ny * g.nz, out3);
+  return hipGetLastError();
+}
+
+hipError_t launch_fdm_substep(const GridView& g, const double* Tin, double* Tout, double inner_dt,
+                              bool any_tdyn, hipStream_t s) {
+  const int n = g.nx * g.ny * g.nz;
+  if (any_tdyn) k_fdm_substep<true><<<nblk(n), kBlock, 0, s>>>(g, Tin, Tout, inner_dt);
+  else k_fdm_substep<false><<<nblk(n), kBlock, 0, s>>>(g, Tin, Tout, inner_dt);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_lookup(const GridView& g, int n, const double* x, const double* y,
+                              const double* z, int* idx, double* T, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_grid_lookup<<<nblk(n), kBlock, 0, s>>>(g, n, x, y, z, idx, T);
+  return hipGetLastError();
+}
+
+hipError_t launch_sum(const double* v, int n, double* out, double* scratch, hipStream_t s) {
+  const int nb = 256;
+  k_sum_partial<<<nb, kBlock, 0, s>>>(v, n, scratch);
+  k_sum_final<<<1, kBlock, 0, s>>>(scratch, nb, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_nonfinite(const double* v, int n, int* flag, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_nonfinite<<<nblk(n), kBlock, 0, s>>>(v, n, flag);
+  return hipGetLastError();
+}
+
+hipError_t launch_peratom(const AtomsView& a, const TablesView& t, const WorkView& w, double* out,
+                          hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  k_peratom<<<nblk(a.nlocal), kBlock, 0, s>>>(a, t, w, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_lj(const AtomsView& a, const int* nbr, const int* nn, int stride, double eps,
+                     double sigma, double rc2, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  const double s6 = sigma * sigma * sigma * sigma * sigma * sigma;
+  const double lj1 = 48.0 * eps * s6 * s6, lj2 = 24.0 * eps * s6;
+  k_lj<<<nblk(a.nlocal), kBlock, 0, s>>>(a, nbr, nn, stride, lj1, lj2, rc2);
+  return hipGetLastError();
+}
+
+hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* partial, int nblocks,
+                            hipStream_t s) {
+  k_max_disp2<<<nblocks, kBlock, 0, s>>>(a, ref, partial);
+  return hipGetLastError();
+}
+
+hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_rows_into_d4<<<nblk(n), kBlock, 0, s>>>(rows, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_d4_into_rows<<<nblk(n), kBlock, 0, s>>>(in, n, rows);
+  return hipGetLastError();
+}
+hipError_t launch_rows_into_soa(const double* rows, int n, double* soa, int stride, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_rows_into_soa<<<nblk(n), kBlock, 0, s>>>(rows, n, soa, stride);
+  return hipGetLastError();
+}
+hipError_t launch_soa_into_rows(const double* soa, int n, int stride, double* rows, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_soa_into_rows<<<nblk(n), kBlock, 0, s>>>(soa, n, stride, rows);
+  return hipGetLastError();
+}
+hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_d4_component<<<nblk(n), kBlock, 0, s>>>(in, n, comp, out);
+  return hipGetLastError();
+}
+
+}  // namespace eph

@@ -1,0 +1,508 @@
+"""Python binding of the MI355X-native `fix eph` library (libeph_amd.so).
+
+The library is the product: every hook runs as gfx950 HIP kernels behind the
+C ABI of include/eph_amd.h.  There is no CPU fallback -- if the shared library
+is missing or no HIP device is present, the calls raise.
+
+    FixEPH  -- the fix (FixEPH hooks, fix_eph.cpp), host-array or device-resident
+    Grid    -- the electronic T_e grid (EPH_FDM, eph_fdm.h), also standalone
+    MD      -- the minimal MD harness standing in for LAMMPS (include/eph_md.h)
+"""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.dirname(HERE)
+LIB_PATH = os.path.join(HERE, "libeph_amd.so")
+
+EPH_OK = 0
+STATUS = {0: "ok", -1: "bad argument", -2: "file", -3: "format", -4: "out of memory",
+          -5: "HIP error", -6: "state", -7: "non-finite T_e", -8: "unsupported", -9: "no device"}
+FRICTION, RANDOM, FDM, NOINT, NOFRICTION, NORANDOM = 0x01, 0x02, 0x04, 0x08, 0x10, 0x20
+
+dp = C.POINTER(C.c_double)
+ip = C.POINTER(C.c_int)
+vp = C.c_void_p
+
+
+class EphError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{STATUS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class MDConfig(C.Structure):
+    _fields_ = [("cells", C.c_int), ("a", C.c_double), ("ntypes", C.c_int), ("seed", C.c_uint),
+                ("T", C.c_double), ("disp_sigma", C.c_double), ("pka_eV", C.c_double),
+                ("mass", C.c_double), ("cut_fix", C.c_double), ("cut_lj", C.c_double),
+                ("skin", C.c_double), ("lj_eps", C.c_double), ("lj_sigma", C.c_double)]
+
+
+_SIG = {
+    "eph_create": (C.c_int, [C.POINTER(vp), C.c_int, C.POINTER(C.c_char_p), C.c_int, dp, dp,
+                             C.c_double, C.c_double, C.c_double, C.c_int]),
+    "eph_destroy": (None, [vp]),
+    "eph_last_error": (C.c_char_p, [vp]),
+    "eph_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp]),
+    "eph_setmask": (C.c_int, []),
+    "eph_reset_dt": (C.c_int, [vp, C.c_double]),
+    "eph_set_masses": (C.c_int, [vp, dp, C.c_int]),
+    "eph_set_groupbit": (C.c_int, [vp, C.c_int]),
+    "eph_get_params": (C.c_int, [vp, ip, ip, ip, dp, dp, dp, dp]),
+    "eph_set_atoms": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, ip, ip, ip]),
+    "eph_set_ghost_map": (C.c_int, [vp, C.c_int, ip, dp]),
+    "eph_set_neighbors": (C.c_int, [vp, C.c_int, ip, ip, C.c_int]),
+    "eph_set_noise": (C.c_int, [vp, dp]),
+    "eph_set_timestep": (C.c_int, [vp, C.c_longlong]),
+    "eph_initial_integrate": (C.c_int, [vp, dp, dp, dp]),
+    "eph_post_force": (C.c_int, [vp, dp, dp, dp]),
+    "eph_final_integrate": (C.c_int, [vp, dp, dp]),
+    "eph_end_of_step": (C.c_int, [vp, dp, dp, C.c_longlong]),
+    "eph_post_run": (C.c_int, [vp]),
+    "eph_compute_vector": (C.c_int, [vp, C.c_int, dp]),
+    "eph_get_peratom": (C.c_int, [vp, dp]),
+    "eph_get_state": (C.c_int, [vp, C.c_int, dp]),
+    "eph_last_energy": (C.c_int, [vp, dp]),
+    "eph_rho_cutoff_warning": (C.c_int, [vp]),
+    "eph_dev_initial_integrate": (C.c_int, [vp]),
+    "eph_dev_post_force": (C.c_int, [vp]),
+    "eph_dev_final_integrate": (C.c_int, [vp]),
+    "eph_dev_end_of_step": (C.c_int, [vp, C.c_longlong]),
+    "eph_dev_get_atoms": (C.c_int, [vp, dp, dp, dp]),
+    "eph_sync": (C.c_int, [vp]),
+    "eph_stream": (vp, [vp]),
+    "eph_timing_enable": (C.c_int, [vp, C.c_int]),
+    "eph_timing_report": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, dp, C.POINTER(C.c_longlong)]),
+    "eph_spline_build": (C.c_int, [C.c_double, dp, C.c_int, dp]),
+    "eph_beta_table": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp]),
+    "eph_grid_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp, C.c_double,
+                                  C.c_double, C.c_double, C.c_double, C.c_int]),
+    "eph_grid_load": (C.c_int, [C.POINTER(vp), C.c_char_p, C.c_int]),
+    "eph_grid_destroy": (None, [vp]),
+    "eph_grid_last_error": (C.c_char_p, [vp]),
+    "eph_context_grid": (vp, [vp]),
+    "eph_grid_set_dt": (C.c_int, [vp, C.c_double]),
+    "eph_grid_set_steps": (C.c_int, [vp, C.c_int]),
+    "eph_grid_set_cell": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
+                                    C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]),
+    "eph_grid_insert_energy": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+    "eph_grid_solve": (C.c_int, [vp, ip]),
+    "eph_grid_get": (C.c_int, [vp, C.c_int, dp]),
+    "eph_grid_set": (C.c_int, [vp, C.c_int, dp]),
+    "eph_grid_lookup": (C.c_int, [vp, C.c_int, dp, dp, dp, ip, dp]),
+    "eph_grid_info": (C.c_int, [vp, ip, dp, dp]),
+    "eph_grid_T_mean": (C.c_int, [vp, dp]),
+    "eph_grid_save_state": (C.c_int, [vp, C.c_char_p]),
+    "eph_grid_save_temperature": (C.c_int, [vp, C.c_char_p, C.c_int]),
+    "eph_md_create_fcc": (C.c_int, [C.POINTER(vp), C.POINTER(MDConfig)]),
+    "eph_md_destroy": (None, [vp]),
+    "eph_md_last_error": (C.c_char_p, [vp]),
+    "eph_md_info": (C.c_int, [vp, ip, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong), dp, dp]),
+    "eph_md_get": (C.c_int, [vp, dp, dp, ip, ip, ip, ip, dp, ip, ip]),
+    "eph_md_update_ghosts": (C.c_int, [vp, dp, dp]),
+    "eph_md_lj_host": (C.c_int, [vp, dp, dp]),
+    "eph_md_needs_rebuild": (C.c_int, [vp, dp]),
+    "eph_md_rebuild": (C.c_int, [vp, dp, dp]),
+    "eph_md_attach": (C.c_int, [vp, vp]),
+    "eph_md_run": (C.c_int, [vp, vp, C.c_int, C.c_longlong, dp, ip]),
+}
+
+_lib = None
+
+
+def build():
+    """Compile libeph_amd.so for gfx950 (hipcc)."""
+    subprocess.check_call(["make", "-s", "-C", PKG, "-j8"])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `make -C user-eph_amd` "
+                               "(the HIP library is required; there is no CPU fallback)")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIG.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _d(a):
+    return a.ctypes.data_as(dp)
+
+
+def _i(a):
+    return a.ctypes.data_as(ip)
+
+
+def f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def i32(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def spline_build(dx, y):
+    """Host-built global-x Akima cubic (EPH_Spline) as [n][4] coefficients."""
+    y = f64(y)
+    out = np.zeros((len(y), 4))
+    rc = lib().eph_spline_build(dx, _d(y), len(y), _d(out))
+    if rc != EPH_OK:
+        raise EphError(rc, lib().eph_last_error(None).decode())
+    return out
+
+
+def beta_table(path, elem, which):
+    """Host-built table of a .beta file: (coeffs [n][4], inv_dx, r_cutoff, rho_cutoff)."""
+    L = lib()
+    n = L.eph_beta_table(path.encode(), elem, which, None, None, None, None)
+    if n < 0:
+        raise EphError(n, L.eph_last_error(None).decode())
+    out = np.zeros((n, 4))
+    inv, rc, rho = C.c_double(), C.c_double(), C.c_double()
+    L.eph_beta_table(path.encode(), elem, which, _d(out), C.byref(inv), C.byref(rc), C.byref(rho))
+    return out, inv.value, rc.value, rho.value
+
+
+class FixEPH:
+    """`fix ID group eph ...` on one GPU (FixEPH, fix_eph.cpp)."""
+
+    def __init__(self, args, ntypes, boxlo, boxhi, dt, boltz, ftm2v, device=0, masses=None):
+        L = lib()
+        arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+        h = vp()
+        lo, hi = f64(boxlo), f64(boxhi)
+        rc = L.eph_create(C.byref(h), len(args), arr, ntypes, _d(lo), _d(hi), dt, boltz, ftm2v, device)
+        if rc != EPH_OK:
+            raise EphError(rc, L.eph_last_error(None).decode())
+        self.h = h
+        self.ntypes = ntypes
+        self.nlocal = self.nghost = 0
+        if masses is not None:
+            self.set_masses(masses)
+
+    def _check(self, rc):
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_last_error(self.h).decode())
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().eph_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def init(self, dimension=3, nonperiodic=0, triclinic=0):
+        c = C.c_double()
+        self._check(lib().eph_init(self.h, dimension, nonperiodic, triclinic, C.byref(c)))
+        return c.value
+
+    def params(self):
+        fl, mo, se = C.c_int(), C.c_int(), C.c_int()
+        rc, rho, eta, dtf = C.c_double(), C.c_double(), C.c_double(), C.c_double()
+        self._check(lib().eph_get_params(self.h, C.byref(fl), C.byref(mo), C.byref(se), C.byref(rc),
+                                         C.byref(rho), C.byref(eta), C.byref(dtf)))
+        return dict(flags=fl.value, model=mo.value, seed=se.value, r_cutoff=rc.value,
+                    rho_cutoff=rho.value, eta_factor=eta.value, dtf=dtf.value)
+
+    def reset_dt(self, dt): self._check(lib().eph_reset_dt(self.h, dt))
+
+    def set_masses(self, masses):
+        m = f64(np.concatenate([[0.0], np.asarray(masses, dtype=np.float64)]))
+        self._check(lib().eph_set_masses(self.h, _d(m), self.ntypes))
+
+    def set_groupbit(self, gb): self._check(lib().eph_set_groupbit(self.h, gb))
+
+    def set_atoms(self, s):
+        """s: object with nlocal, nghost, x, v ([nall][3]), type, mask, tag"""
+        self.nlocal, self.nghost = s.nlocal, s.nghost
+        self._check(lib().eph_set_atoms(self.h, s.nlocal, s.nghost, _d(f64(s.x)), _d(f64(s.v)),
+                                        _i(i32(s.type)), _i(i32(s.mask)), _i(i32(s.tag))))
+        sh = f64(s.ghost_shift) if s.nghost else np.zeros((0, 3))
+        self._check(lib().eph_set_ghost_map(self.h, s.nghost, _i(i32(s.ghost_owner)), _d(sh)))
+        self._check(lib().eph_set_neighbors(self.h, s.nlocal, _i(i32(s.nbr_off)), _i(i32(s.nbr_idx)), -1))
+
+    def set_noise(self, xi):
+        if xi is None:
+            self._check(lib().eph_set_noise(self.h, None))
+        else:
+            xi = f64(xi)
+            self._check(lib().eph_set_noise(self.h, _d(xi)))
+
+    def set_timestep(self, n): self._check(lib().eph_set_timestep(self.h, n))
+
+    # host-array hooks (arrays updated in place)
+    def initial_integrate(self, s):
+        self._check(lib().eph_initial_integrate(self.h, _d(s.x), _d(s.v), _d(s.f)))
+
+    def post_force(self, s, xi=None, step=None):
+        if step is not None:
+            self.set_timestep(step)
+        if xi is not None:
+            self.set_noise(xi)
+        self._check(lib().eph_post_force(self.h, _d(s.x), _d(s.v), _d(s.f)))
+
+    def final_integrate(self, s):
+        self._check(lib().eph_final_integrate(self.h, _d(s.v), _d(s.f)))
+
+    def end_of_step(self, s, ntimestep=0):
+        self._check(lib().eph_end_of_step(self.h, _d(s.x), _d(s.v), ntimestep))
+
+    def post_run(self): self._check(lib().eph_post_run(self.h))
+
+    def compute_vector(self, i):
+        out = C.c_double()
+        self._check(lib().eph_compute_vector(self.h, i, C.byref(out)))
+        return out.value
+
+    def last_energy(self):
+        out = C.c_double()
+        self._check(lib().eph_last_energy(self.h, C.byref(out)))
+        return out.value
+
+    def rho_cutoff_warning(self): return bool(lib().eph_rho_cutoff_warning(self.h))
+
+    def peratom(self):
+        out = np.zeros((self.nlocal, 8))
+        self._check(lib().eph_get_peratom(self.h, _d(out)))
+        return out
+
+    def get(self, which):
+        idx = {"rho": 0, "w": 1, "xi": 2, "f_EPH": 3, "f_RNG": 4, "alpha": 5, "nactive": 6,
+               "nlist": 7}[which]
+        nall = self.nlocal + self.nghost
+        shape = {0: (nall,), 5: (nall,), 1: (nall, 3), 2: (nall, 3), 6: (self.nlocal,),
+                 7: (self.nlocal,)}.get(idx, (self.nlocal, 3))
+        out = np.zeros(shape)
+        self._check(lib().eph_get_state(self.h, idx, _d(out)))
+        return out
+
+    # device-resident hooks
+    def dev_initial_integrate(self): self._check(lib().eph_dev_initial_integrate(self.h))
+    def dev_post_force(self): self._check(lib().eph_dev_post_force(self.h))
+    def dev_final_integrate(self): self._check(lib().eph_dev_final_integrate(self.h))
+    def dev_end_of_step(self, n): self._check(lib().eph_dev_end_of_step(self.h, n))
+    def sync(self): self._check(lib().eph_sync(self.h))
+    def stream(self): return lib().eph_stream(self.h)
+
+    def dev_atoms(self):
+        nall = self.nlocal + self.nghost
+        x, v, f = np.zeros((nall, 3)), np.zeros((nall, 3)), np.zeros((self.nlocal, 3))
+        self._check(lib().eph_dev_get_atoms(self.h, _d(x), _d(v), _d(f)))
+        return x, v, f
+
+    def timing(self, on=True): self._check(lib().eph_timing_enable(self.h, 1 if on else 0))
+
+    def timing_report(self):
+        L = lib()
+        n = L.eph_timing_report(self.h, -1, None, 0, None, None)
+        if n < 0:
+            self._check(n)
+        res = {}
+        for k in range(n):
+            name = C.create_string_buffer(64)
+            ms = C.c_double()
+            cnt = C.c_longlong()
+            self._check(L.eph_timing_report(self.h, k, name, 64, C.byref(ms), C.byref(cnt)))
+            res[name.value.decode()] = (ms.value, cnt.value)
+        return res
+
+    def grid(self):
+        return Grid(lib().eph_context_grid(self.h), owner=self)
+
+
+class Grid:
+    """Electronic temperature grid (EPH_FDM, eph_fdm.h)."""
+
+    def __init__(self, h, owner=None):
+        self.h = h
+        self.owner = owner  # keeps the fix alive when the grid belongs to it
+        dims = (C.c_int * 3)()
+        lib().eph_grid_info(self.h, dims, None, None)
+        self.dims = (dims[0], dims[1], dims[2])
+
+    @classmethod
+    def const(cls, nx, ny, nz, box, T, C_e, rho, kappa, device=0):
+        h = vp()
+        b = f64(box)
+        rc = lib().eph_grid_create(C.byref(h), nx, ny, nz, _d(b), T, C_e, rho, kappa, device)
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_grid_last_error(None).decode())
+        return cls(h)
+
+    @classmethod
+    def load(cls, path, device=0):
+        h = vp()
+        rc = lib().eph_grid_load(C.byref(h), path.encode(), device)
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_grid_last_error(None).decode())
+        return cls(h)
+
+    def _check(self, rc):
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_grid_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None) and self.owner is None:
+            lib().eph_grid_destroy(self.h)
+        self.h = None
+
+    __del__ = close
+
+    @property
+    def ntotal(self):
+        return self.dims[0] * self.dims[1] * self.dims[2]
+
+    def set_dt(self, dt): self._check(lib().eph_grid_set_dt(self.h, dt))
+    def set_steps(self, s): self._check(lib().eph_grid_set_steps(self.h, s))
+
+    def set_cell(self, i, j, k, T, S, rho, Ce, kappa, flag, tdyn):
+        self._check(lib().eph_grid_set_cell(self.h, i, j, k, T, S, rho, Ce, kappa, flag, tdyn))
+
+    def insert_energy(self, x, y, z, E):
+        x, y, z, E = (f64(np.atleast_1d(a)) for a in (x, y, z, E))
+        self._check(lib().eph_grid_insert_energy(self.h, len(x), _d(x), _d(y), _d(z), _d(E)))
+
+    def solve(self):
+        n = C.c_int()
+        self._check(lib().eph_grid_solve(self.h, C.byref(n)))
+        return n.value
+
+    def last_r(self):
+        r = C.c_double()
+        lib().eph_grid_info(self.h, None, None, C.byref(r))
+        return r.value
+
+    def get(self, which="T"):
+        idx = {"T": 0, "dT": 1, "C": 2, "rho": 3, "kappa": 4, "S": 5}[which]
+        out = np.zeros(self.ntotal)
+        self._check(lib().eph_grid_get(self.h, idx, _d(out)))
+        return out
+
+    def set(self, which, values):
+        idx = {"T": 0, "dT": 1, "C": 2, "rho": 3, "kappa": 4, "S": 5}[which]
+        v = f64(values)
+        self._check(lib().eph_grid_set(self.h, idx, _d(v)))
+
+    def lookup(self, x, y, z):
+        x, y, z = (f64(np.atleast_1d(a)) for a in (x, y, z))
+        idx = np.zeros(len(x), np.int32)
+        T = np.zeros(len(x))
+        self._check(lib().eph_grid_lookup(self.h, len(x), _d(x), _d(y), _d(z), _i(idx), _d(T)))
+        return idx, T
+
+    def T_mean(self):
+        out = C.c_double()
+        self._check(lib().eph_grid_T_mean(self.h, C.byref(out)))
+        return out.value
+
+    def save_state(self, path): self._check(lib().eph_grid_save_state(self.h, path.encode()))
+
+    def save_temperature(self, base, n):
+        self._check(lib().eph_grid_save_temperature(self.h, base.encode(), n))
+
+
+class MDSystem:
+    """Host snapshot of the harness system in LAMMPS layout."""
+    pass
+
+
+class MD:
+    """Minimal MD harness (include/eph_md.h): fcc lattice, ghosts, lists, LJ."""
+
+    def __init__(self, cells, a=3.615, ntypes=1, seed=1, T=300.0, disp_sigma=0.0, pka_eV=0.0,
+                 mass=63.546, cut_fix=4.5, cut_lj=5.5, skin=2.0, lj_eps=0.4, lj_sigma=2.3):
+        self.cfg = MDConfig(cells, a, ntypes, seed & 0xFFFFFFFF, T, disp_sigma, pka_eV, mass,
+                            cut_fix, cut_lj, skin, lj_eps, lj_sigma)
+        h = vp()
+        rc = lib().eph_md_create_fcc(C.byref(h), C.byref(self.cfg))
+        self.h = h
+        if rc != EPH_OK:
+            msg = lib().eph_md_last_error(h).decode() if h else ""
+            raise EphError(rc, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().eph_md_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def info(self):
+        nl, ng = C.c_int(), C.c_int()
+        nf, nj = C.c_longlong(), C.c_longlong()
+        lo, hi = np.zeros(3), np.zeros(3)
+        lib().eph_md_info(self.h, C.byref(nl), C.byref(ng), C.byref(nf), C.byref(nj), _d(lo), _d(hi))
+        return dict(nlocal=nl.value, nghost=ng.value, nfix=nf.value, nlj=nj.value, boxlo=lo, boxhi=hi)
+
+    def snapshot(self):
+        inf = self.info()
+        nl, ng = inf["nlocal"], inf["nghost"]
+        nall = nl + ng
+        s = MDSystem()
+        s.nlocal, s.nghost = nl, ng
+        s.boxlo, s.boxhi = inf["boxlo"], inf["boxhi"]
+        s.x, s.v = np.zeros((nall, 3)), np.zeros((nall, 3))
+        s.type, s.mask, s.tag = (np.zeros(nall, np.int32) for _ in range(3))
+        s.ghost_owner = np.zeros(ng, np.int32)
+        s.ghost_shift = np.zeros((ng, 3))
+        s.nbr_off = np.zeros(nl + 1, np.int32)
+        s.nbr_idx = np.zeros(inf["nfix"], np.int32)
+        lib().eph_md_get(self.h, _d(s.x), _d(s.v), _i(s.type), _i(s.mask), _i(s.tag),
+                         _i(s.ghost_owner), _d(s.ghost_shift), _i(s.nbr_off), _i(s.nbr_idx))
+        s.f = np.zeros((nall, 3))
+        return s
+
+    def update_ghosts(self, s):
+        lib().eph_md_update_ghosts(self.h, _d(s.x), _d(s.v))
+
+    def lj(self, s):
+        """pair forces into s.f (all local rows overwritten)"""
+        f = np.zeros((s.nlocal, 3))
+        lib().eph_md_lj_host(self.h, _d(s.x), _d(f))
+        s.f[:s.nlocal] = f
+
+    def needs_rebuild(self, s):
+        return bool(lib().eph_md_needs_rebuild(self.h, _d(f64(s.x[:s.nlocal]))))
+
+    def rebuild(self, s):
+        x = f64(s.x[:s.nlocal]); v = f64(s.v[:s.nlocal])
+        rc = lib().eph_md_rebuild(self.h, _d(x), _d(v))
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_md_last_error(self.h).decode())
+
+    def attach(self, fix):
+        rc = lib().eph_md_attach(self.h, fix.h)
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_md_last_error(self.h).decode())
+        inf = self.info()
+        fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
+
+    def run(self, fix, nsteps, first_step=1):
+        t = np.zeros(4)
+        nreb = C.c_int()
+        rc = lib().eph_md_run(self.h, fix.h, nsteps, first_step, _d(t), C.byref(nreb))
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_md_last_error(self.h).decode())
+        inf = self.info()
+        fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
+        return dict(fix_ms=t[0], total_ms=t[1], pair_ms=t[2], rebuild_ms=t[3], rebuilds=nreb.value)
+
+
+def exported_symbols(header_paths):
+    """Function names declared in the C headers (for the export check)."""
+    import re
+    names = []
+    for p in header_paths:
+        text = open(p).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        names += re.findall(r"\b(eph_[a-z0-9_]+)\s*\(", text)
+    return sorted(set(names))
