@@ -90,6 +90,9 @@ def main():
     ap.add_argument("--cells", type=int, default=80)
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="also run a plain device copy of MIB MiB (torch) three times, to scale "
+                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -145,8 +148,11 @@ def main():
             for k in res:
                 res[k] += r[k]
             if blowup_step is None:
+                # every cell starts at >= 300 K and only gains or diffuses heat; a
+                # cell clamped at 0 K (eph_fdm.h:391-394) or a non-finite value is
+                # the unstable explicit stencil (one sub-step at lambda 0.255)
                 T = g.get("T")
-                if not np.all(np.isfinite(T)) or T.max() > 2.0 * T0max:
+                if not np.all(np.isfinite(T)) or T.min() <= 0.0:
                     blowup_step = step
             step += 1
         return res
@@ -213,6 +219,12 @@ def main():
         out["cpu_baseline"] = {"value": v_cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
                                "sample": f"{args.cpu_steps} fix steps of the same {N}-atom system "
                                          f"({t_cpu:.1f} s), scalar oracle on one host core"}
+    if args.calibrate_copy > 0:
+        # plain copy kernel (k_copy_probe) of a known size, in this same process
+        nbytes = args.calibrate_copy * (1 << 20)
+        ms = ea.copy_probe(nbytes, 3)
+        out["calibration_copy"] = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": ms,
+                                   "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
     print(json.dumps(out), flush=True)
 
 

@@ -156,6 +156,12 @@ int eph_timing_enable(eph_context* ctx, int on);
 int eph_timing_report(eph_context* ctx, int k, char* name, int namelen, double* total_ms,
                       long long* count);
 
+/* Plain device-to-device copy of `bytes` bytes (double4 loads/stores), run
+ * `reps` times on a fresh buffer pair; *ms receives the average kernel time.
+ * No counterpart in the reference: it calibrates the HBM byte counters of a
+ * profiled run and measures the achievable copy bandwidth. */
+int eph_copy_probe(long long bytes, int reps, int device, double* ms);
+
 /* ------------------------------------------------ host-side table checks */
 /* Host construction only, no device needed.
  * EPH_Spline(dx, y) of eph_spline.h:31-132: coeffs [n][4] = a, b, c, d of the

@@ -879,6 +879,36 @@ int eph_timing_report(eph_context* ctx, int k, char* name, int namelen, double* 
   return EPH_OK;
 }
 
+int eph_copy_probe(long long bytes, int reps, int device, double* ms) {
+  if (bytes < 32 || reps < 1) return EPH_ERR_ARGS;
+  if (hipSetDevice(device) != hipSuccess) return EPH_ERR_NO_DEVICE;
+  const long n = static_cast<long>(bytes / 32);
+  double4 *a = nullptr, *b = nullptr;
+  hipStream_t s = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  int rc = EPH_OK;
+  float t = 0.f;
+  if (hipMalloc(reinterpret_cast<void**>(&a), n * 32) || hipMalloc(reinterpret_cast<void**>(&b), n * 32)) {
+    rc = EPH_ERR_OOM;
+  } else if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) || hipEventCreate(&e0) ||
+             hipEventCreate(&e1) || hipMemsetAsync(a, 0, n * 32, s) ||
+             launch_copy_probe(b, a, n, s) || hipEventRecord(e0, s)) {
+    rc = EPH_ERR_HIP;
+  } else {
+    for (int r = 0; r < reps && rc == EPH_OK; ++r)
+      if (launch_copy_probe(b, a, n, s)) rc = EPH_ERR_HIP;
+    if (rc == EPH_OK && (hipEventRecord(e1, s) || hipEventSynchronize(e1) || hipEventElapsedTime(&t, e0, e1)))
+      rc = EPH_ERR_HIP;
+  }
+  if (ms) *ms = t / reps;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (s) (void)hipStreamDestroy(s);
+  if (a) (void)hipFree(a);
+  if (b) (void)hipFree(b);
+  return rc;
+}
+
 // ========================================================= host table checks
 int eph_spline_build(double dx, const double* y, int n, double* coeffs) {
   if (!y || !coeffs || n < 4) return EPH_ERR_ARGS;

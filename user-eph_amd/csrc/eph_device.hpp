@@ -158,5 +158,6 @@ hipError_t launch_lj(const AtomsView& a, const int* nbr, const int* nn, int stri
                      double sigma, double rc2, hipStream_t s);
 hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* partial, int nblocks,
                             hipStream_t s);
+hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream_t s);
 
 }  // namespace eph

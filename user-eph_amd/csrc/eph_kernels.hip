@@ -993,6 +993,18 @@ hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* part
   return hipGetLastError();
 }
 
+__global__ __launch_bounds__(kBlock) void k_copy_probe(double4* __restrict__ dst,
+                                                       const double4* __restrict__ src, long n) {
+  for (long i = static_cast<long>(blockIdx.x) * kBlock + threadIdx.x; i < n;
+       i += static_cast<long>(gridDim.x) * kBlock)
+    dst[i] = src[i];
+}
+
+hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream_t s) {
+  k_copy_probe<<<4096, kBlock, 0, s>>>(dst, src, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_rows_into_d4<<<nblk(n), kBlock, 0, s>>>(rows, n, out);

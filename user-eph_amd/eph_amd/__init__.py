@@ -76,6 +76,7 @@ _SIG = {
     "eph_stream": (vp, [vp]),
     "eph_timing_enable": (C.c_int, [vp, C.c_int]),
     "eph_timing_report": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, dp, C.POINTER(C.c_longlong)]),
+    "eph_copy_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, dp]),
     "eph_spline_build": (C.c_int, [C.c_double, dp, C.c_int, dp]),
     "eph_beta_table": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp]),
     "eph_grid_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp, C.c_double,
@@ -147,6 +148,15 @@ def f64(a):
 
 def i32(a):
     return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def copy_probe(nbytes, reps=3, device=0):
+    """Average ms of a plain device copy of nbytes (read) + nbytes (write)."""
+    ms = C.c_double()
+    rc = lib().eph_copy_probe(int(nbytes), reps, device, C.byref(ms))
+    if rc != EPH_OK:
+        raise EphError(rc, "copy probe failed")
+    return ms.value
 
 
 def spline_build(dx, y):
