@@ -70,6 +70,14 @@ int eph_md_attach(eph_md* md, eph_context* ctx);
  *   t[3] host rebuilds (count in *nrebuild). */
 int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, double* t,
                int* nrebuild);
+/* rebuilds on the device (1, default; falls back to the host when a capacity
+ * is exceeded) or through the host (0); both give the same ghosts and lists */
+int eph_md_set_device_rebuild(eph_md* md, int on);
+/* rebuild at the next step whatever the displacements */
+int eph_md_force_rebuild(eph_md* md);
+/* copy the device-resident system (atoms, ghost map, both lists) back into the
+ * harness's host arrays, so eph_md_info / eph_md_get describe it */
+int eph_md_pull(eph_md* md, eph_context* ctx);
 
 #ifdef __cplusplus
 }

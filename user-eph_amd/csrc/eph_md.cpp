@@ -9,6 +9,7 @@
 #include <thread>
 
 #include "eph_context.hpp"
+#include "eph_nbuild.hpp"
 #include "eph_rng.hpp"
 
 extern "C" {
@@ -58,11 +59,22 @@ struct eph_md {
   DevArray<double4> xref;
   DevArray<double> partial;
   int lj_stride = 0;
+  // device rebuild (eph_nbuild.hip)
+  bool device_rebuild = true;
+  bool force_next = false;
+  int ghost_cap = 0, cap_f = 0, cap_l = 0, nbins = 0;
+  DevArray<int> rep, flags, cnt, start, bin_of, bcount, bstart, bfill, items, scan_tmp, nbi;
+  DevArray<double> box_d;
+  NbBuild nbw;
+  int* pinned = nullptr;  // go, nghost, overflow
 
   int fail(int code, const std::string& m) { err = m; return code; }
   int fail_hip(hipError_t e, const char* what) {
     err = std::string("HIP error ") + hipGetErrorString(e) + " in " + what;
     return EPH_ERR_HIP;
+  }
+  ~eph_md() {
+    if (pinned) (void)hipHostFree(pinned);
   }
   void wrap();
   void build_ghosts();
@@ -196,13 +208,26 @@ void eph_md::build_lists() {
 
 int eph_md::upload(eph_context* ctx) {
   int rc;
+  int mxf = 0, mx = 0;
+  for (int i = 0; i < nlocal; ++i) {
+    mxf = std::max(mxf, fix.off[i + 1] - fix.off[i]);
+    mx = std::max(mx, lj.off[i + 1] - lj.off[i]);
+  }
+  // capacities for device rebuilds: ghosts and list widths with headroom
+  ghost_cap = std::max(ghost_cap, nghost + nghost / 4 + 1024);
+  cap_f = std::max(cap_f, mxf + mxf / 4 + 16);
+  cap_l = std::max(cap_l, mx + mx / 4 + 16);
+  if ((rc = ctx->reserve_atoms(nlocal, ghost_cap))) return fail(rc, ctx->err);
+  EPH_HIP(this, ctx->nbr.reserve((size_t)cap_f * ctx->stride));
+  EPH_HIP(this, ctx->act.reserve((size_t)cap_f * ctx->stride));
+  EPH_HIP(this, ctx->ghost_owner.reserve(ghost_cap));
+  EPH_HIP(this, ctx->ghost_shift.reserve(ghost_cap));
   if ((rc = ctx->set_atoms(nlocal, nghost, x.data(), v.data(), type.data(), mask.data(), tag.data())))
     return fail(rc, ctx->err);
   if ((rc = ctx->set_ghost_map(nghost, owner.data(), shift.data()))) return fail(rc, ctx->err);
   if ((rc = ctx->set_neighbors(nlocal, fix.off.data(), fix.idx.data(), ~0))) return fail(rc, ctx->err);
   // lj list in the context's ELL stride
-  int mx = 0;
-  for (int i = 0; i < nlocal; ++i) mx = std::max(mx, lj.off[i + 1] - lj.off[i]);
+  mx = cap_l;
   lj_stride = ctx->stride;
   hipStream_t s = ctx->stream;
   EPH_HIP(this, lj_nbr.reserve((size_t)std::max(mx, 1) * lj_stride));
@@ -216,7 +241,91 @@ int eph_md::upload(eph_context* ctx) {
   EPH_HIP(this, xref.reserve(nlocal));
   EPH_HIP(this, hipMemcpyAsync(xref.p, ctx->pos.p, nlocal * sizeof(double4), hipMemcpyDeviceToDevice, s));
   EPH_HIP(this, partial.reserve(1024));
+  // device rebuild workspace (single system: replica 0)
+  const double cf = cfg.cut_fix + cfg.skin, cl = cfg.cut_lj + cfg.skin, cmax = std::max(cf, cl);
+  const double bs = 0.5 * cmax;
+  int nb[3];
+  for (int d = 0; d < 3; ++d) nb[d] = std::max(1, (int)std::ceil((L[d] + 2 * cmax) / bs) + 1);
+  nbins = nb[0] * nb[1] * nb[2];
+  const size_t cap = (size_t)nlocal + ghost_cap;
+  EPH_HIP(this, rep.reserve(cap));
+  EPH_HIP(this, hipMemsetAsync(rep.p, 0, cap * sizeof(int), s));
+  EPH_HIP(this, flags.reserve(8));
+  EPH_HIP(this, hipMemsetAsync(flags.p, 0, 8 * sizeof(int), s));
+  EPH_HIP(this, cnt.reserve(cap));
+  EPH_HIP(this, start.reserve(cap + 1));
+  EPH_HIP(this, bin_of.reserve(cap));
+  EPH_HIP(this, items.reserve(cap));
+  EPH_HIP(this, bcount.reserve(nbins + 1));
+  EPH_HIP(this, bstart.reserve(nbins + 1));
+  EPH_HIP(this, bfill.reserve(nbins + 1));
+  EPH_HIP(this, scan_tmp.reserve((cap + nbins) / 4096 + 4));
+  EPH_HIP(this, nbi.reserve(8));
+  const double bx[9] = {lo[0], lo[1], lo[2], hi[0], hi[1], hi[2], L[0], L[1], L[2]};
+  const double gl[3] = {lo[0] - cmax, lo[1] - cmax, lo[2] - cmax};
+  EPH_HIP(this, box_d.reserve(12));
+  EPH_HIP(this, hipMemcpyAsync(box_d.p, bx, sizeof bx, hipMemcpyHostToDevice, s));
+  EPH_HIP(this, hipMemcpyAsync(box_d.p + 9, gl, sizeof gl, hipMemcpyHostToDevice, s));
+  const int nbv[4] = {nb[0], nb[1], nb[2], 0};
+  EPH_HIP(this, hipMemcpyAsync(nbi.p, nbv, sizeof nbv, hipMemcpyHostToDevice, s));
+  if (!pinned) EPH_HIP(this, hipHostMalloc(reinterpret_cast<void**>(&pinned), 8 * sizeof(int)));
+  NbBuild& w = nbw;
+  w.nlocal = nlocal; w.ghost_cap = ghost_cap; w.nbins = nbins;
+  w.at.pos = ctx->pos.p; w.at.vel = ctx->vel.p; w.at.type = ctx->type.p; w.at.mask = ctx->mask.p;
+  w.at.tag = ctx->tag.p; w.at.rep = rep.p; w.at.owner = ctx->ghost_owner.p; w.at.shift = ctx->ghost_shift.p;
+  w.box.lo = box_d.p; w.box.hi = box_d.p + 3; w.box.L = box_d.p + 6; w.box.glo = box_d.p + 9;
+  w.box.nb = nbi.p; w.box.bin_base = nbi.p + 3; w.box.bs = bs; w.box.cg = cmax;
+  w.lists.nbr_f = ctx->nbr.p; w.lists.nn_f = ctx->nnbr.p; w.lists.cap_f = cap_f; w.lists.cf2 = cf * cf;
+  w.lists.nbr_l = lj_nbr.p; w.lists.nn_l = lj_nn.p; w.lists.cap_l = cap_l; w.lists.cl2 = cl * cl;
+  w.lists.stride = ctx->stride;
+  w.go = flags.p; w.overflow = flags.p + 1; w.nrebuild = flags.p + 2; w.nghost = flags.p + 4;
+  w.cnt = cnt.p; w.start = start.p; w.bin_of = bin_of.p; w.bcount = bcount.p; w.bstart = bstart.p;
+  w.bfill = bfill.p; w.items = items.p; w.scan_tmp = scan_tmp.p; w.xref = xref.p;
   EPH_HIP(this, hipStreamSynchronize(s));
+  return EPH_OK;
+}
+
+// copy the device-built system back into the host arrays (for checking)
+static int md_pull(eph_md* md, eph_context* ctx) {
+  hipStream_t s = ctx->stream;
+  const int nl = ctx->nlocal, ng = ctx->nghost;
+  const size_t nall = (size_t)nl + ng;
+  md->nghost = ng;
+  md->x.resize(3 * nall); md->v.resize(3 * nall);
+  int rc = eph_dev_get_atoms(ctx, md->x.data(), md->v.data(), nullptr);
+  if (rc) return md->fail(rc, ctx->err);
+  md->type.resize(nall); md->mask.resize(nall); md->tag.resize(nall);
+  md->owner.resize(ng); md->shift.resize(3 * (size_t)ng);
+  std::vector<double4> sh(ng);
+  std::vector<int> nnf(nl), nnl(nl), ef((size_t)md->cap_f * ctx->stride), el((size_t)md->cap_l * ctx->stride);
+  EPH_HIP(md, hipMemcpyAsync(md->type.data(), ctx->type.p, nall * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipMemcpyAsync(md->mask.data(), ctx->mask.p, nall * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipMemcpyAsync(md->tag.data(), ctx->tag.p, nall * sizeof(int), hipMemcpyDeviceToHost, s));
+  if (ng) {
+    EPH_HIP(md, hipMemcpyAsync(md->owner.data(), ctx->ghost_owner.p, ng * sizeof(int), hipMemcpyDeviceToHost, s));
+    EPH_HIP(md, hipMemcpyAsync(sh.data(), ctx->ghost_shift.p, ng * sizeof(double4), hipMemcpyDeviceToHost, s));
+  }
+  EPH_HIP(md, hipMemcpyAsync(nnf.data(), ctx->nnbr.p, nl * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipMemcpyAsync(nnl.data(), md->lj_nn.p, nl * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipMemcpyAsync(ef.data(), ctx->nbr.p, ef.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipMemcpyAsync(el.data(), md->lj_nbr.p, el.size() * sizeof(int), hipMemcpyDeviceToHost, s));
+  EPH_HIP(md, hipStreamSynchronize(s));
+  for (int g = 0; g < ng; ++g) {
+    md->shift[3 * (size_t)g] = sh[g].x; md->shift[3 * (size_t)g + 1] = sh[g].y; md->shift[3 * (size_t)g + 2] = sh[g].z;
+  }
+  const int st = ctx->stride;
+  md->fix.off.assign(nl + 1, 0);
+  md->lj.off.assign(nl + 1, 0);
+  for (int i = 0; i < nl; ++i) {
+    md->fix.off[i + 1] = md->fix.off[i] + nnf[i];
+    md->lj.off[i + 1] = md->lj.off[i] + nnl[i];
+  }
+  md->fix.idx.resize(md->fix.off[nl]);
+  md->lj.idx.resize(md->lj.off[nl]);
+  for (int i = 0; i < nl; ++i) {
+    for (int k = 0; k < nnf[i]; ++k) md->fix.idx[md->fix.off[i] + k] = ef[(size_t)k * st + i];
+    for (int k = 0; k < nnl[i]; ++k) md->lj.idx[md->lj.off[i] + k] = el[(size_t)k * st + i];
+  }
   return EPH_OK;
 }
 
@@ -442,6 +551,23 @@ int eph_md_attach(eph_md* md, eph_context* ctx) {
   return md->upload(ctx);
 }
 
+int eph_md_set_device_rebuild(eph_md* md, int on) {
+  if (!md) return EPH_ERR_ARGS;
+  md->device_rebuild = on != 0;
+  return EPH_OK;
+}
+
+int eph_md_force_rebuild(eph_md* md) {
+  if (!md) return EPH_ERR_ARGS;
+  md->force_next = true;
+  return EPH_OK;
+}
+
+int eph_md_pull(eph_md* md, eph_context* ctx) {
+  if (!md || !ctx) return EPH_ERR_ARGS;
+  return md_pull(md, ctx);
+}
+
 int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, double* t,
                int* nrebuild) {
   if (!md || !ctx || nsteps < 0) return EPH_ERR_ARGS;
@@ -460,12 +586,34 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     EPH_HIP(md, hipEventRecord(ev[0], s));
     if ((rc = ctx->dev_initial_integrate())) break;
     EPH_HIP(md, hipEventRecord(ev[1], s));
-    // neighbor check (skin / 2)
-    EPH_HIP(md, launch_max_disp2(ctx->atoms_view(), md->xref.p, md->partial.p, nbp, s));
-    EPH_HIP(md, hipMemcpyAsync(part.data(), md->partial.p, nbp * sizeof(double), hipMemcpyDeviceToHost, s));
-    EPH_HIP(md, hipStreamSynchronize(s));
-    const double mx = *std::max_element(part.begin(), part.end());
-    if (mx > lim) {
+    bool host_rebuild = false;
+    if (md->device_rebuild) {
+      // neighbour check and, when an atom moved more than skin/2, the device
+      // rebuild of ghosts and both lists
+      EPH_HIP(md, launch_nb_check(md->nlocal, ctx->pos.p, md->xref.p, lim, md->nbw.go, s));
+      if (md->force_next) EPH_HIP(md, launch_nb_force(md->nbw.go, s));
+      md->force_next = false;
+      EPH_HIP(md, launch_nb_build(md->nbw, s));
+      EPH_HIP(md, hipMemcpyAsync(md->pinned, md->flags.p, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
+      EPH_HIP(md, hipStreamSynchronize(s));
+      if (md->pinned[0]) {
+        if (md->pinned[1]) {
+          host_rebuild = true;  // capacity exceeded: rebuild through the host
+          EPH_HIP(md, launch_fill_int(md->flags.p + 1, 1, 0, s));
+        } else {
+          ctx->nghost = md->pinned[7];
+          md->nghost = ctx->nghost;
+          ++nreb;
+        }
+      }
+    } else {
+      EPH_HIP(md, launch_max_disp2(ctx->atoms_view(), md->xref.p, md->partial.p, nbp, s));
+      EPH_HIP(md, hipMemcpyAsync(part.data(), md->partial.p, nbp * sizeof(double), hipMemcpyDeviceToHost, s));
+      EPH_HIP(md, hipStreamSynchronize(s));
+      host_rebuild = *std::max_element(part.begin(), part.end()) > lim || md->force_next;
+      md->force_next = false;
+    }
+    if (host_rebuild) {
       const auto t0 = std::chrono::steady_clock::now();
       std::vector<double> xh(3 * (size_t)(ctx->nlocal + ctx->nghost)), vh(xh.size());
       if ((rc = eph_dev_get_atoms(ctx, xh.data(), vh.data(), nullptr))) break;
@@ -491,9 +639,12 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     EPH_HIP(md, hipEventElapsedTime(&b, ev[3], ev[4]));
     EPH_HIP(md, hipEventElapsedTime(&c, ev[2], ev[3]));
     EPH_HIP(md, hipEventElapsedTime(&d, ev[0], ev[4]));
+    float e = 0;
+    if (!host_rebuild) EPH_HIP(md, hipEventElapsedTime(&e, ev[1], ev[2]));
     t_fix += a + b;
     t_pair += c;
     t_total += d;
+    t_rebuild += e;  // device neighbour check / rebuild
   }
   for (auto& e : ev) (void)hipEventDestroy(e);
   if (rc != EPH_OK) {

@@ -109,6 +109,9 @@ _SIG = {
     "eph_md_rebuild": (C.c_int, [vp, dp, dp]),
     "eph_md_attach": (C.c_int, [vp, vp]),
     "eph_md_run": (C.c_int, [vp, vp, C.c_int, C.c_longlong, dp, ip]),
+    "eph_md_set_device_rebuild": (C.c_int, [vp, C.c_int]),
+    "eph_md_force_rebuild": (C.c_int, [vp]),
+    "eph_md_pull": (C.c_int, [vp, vp]),
 }
 
 _lib = None
@@ -495,6 +498,18 @@ class MD:
             raise EphError(rc, lib().eph_md_last_error(self.h).decode())
         inf = self.info()
         fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
+
+    def set_device_rebuild(self, on=True):
+        lib().eph_md_set_device_rebuild(self.h, 1 if on else 0)
+
+    def force_rebuild(self):
+        lib().eph_md_force_rebuild(self.h)
+
+    def pull(self, fix):
+        """copy the device-resident system back so snapshot() describes it"""
+        rc = lib().eph_md_pull(self.h, fix.h)
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_md_last_error(self.h).decode())
 
     def run(self, fix, nsteps, first_step=1):
         t = np.zeros(4)
