@@ -63,11 +63,11 @@ def test_device_rebuild_matches_host_build(tmp_path, cells, species):
 
 
 def test_device_resident_run_matches_oracle(oracle, tmp_path):
-    """fcc 6^3, 1 keV PKA at dt = 1 fs (rebuilds every few steps), 24 steps:
+    """fcc 6^3, 200 eV PKA at dt = 0.5 fs (rebuilds every few steps), 40 steps:
     the device loop (pair, ghosts, device rebuilds, fix hooks, Philox noise)
     against the oracle driven through the host harness."""
-    cells, grid, dt, nsteps = 6, 6, 1e-3, 24
-    cfg = dict(a=3.615, ntypes=1, seed=5, T=300.0, pka_eV=1000.0)
+    cells, grid, dt, nsteps = 6, 6, 5e-4, 40
+    cfg = dict(a=3.615, ntypes=1, seed=5, T=300.0, pka_eV=200.0)
     md_d = ea.MD(cells, **cfg)
     args, inf = _fix(str(tmp_path), md_d, cells=cells, grid=grid)
     fix = ea.FixEPH(args, 1, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,

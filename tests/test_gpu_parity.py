@@ -89,6 +89,15 @@ def test_alloy_cascade_small(oracle, tmp_path):
     _fix_state_close(p)
 
 
+def test_alloy_lds_table_path(oracle, tmp_path):
+    """fcc 21^3 (37,044 atoms), 4 species: large enough for the persistent
+    kernels that stage all four rho(r^2) tables (128 KB) in LDS."""
+    p = Pair(oracle, ea, str(tmp_path), cells=21, species=4, T=900.0, disp=0.05, grid=7, dt=5e-4)
+    for _ in range(3):
+        p.step()
+    _fix_state_close(p)
+
+
 @pytest.mark.parametrize("flags", ["1", "2", "3", "5", "0x17", "0x27", "0xF"])
 def test_flag_combinations(oracle, tmp_path, flags):
     p = Pair(oracle, ea, str(tmp_path), cells=5, T=800.0, disp=0.05, grid=5, flags=flags)

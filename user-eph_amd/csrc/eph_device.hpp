@@ -111,6 +111,11 @@ struct DepositView {
 constexpr int kBlock = 256;
 constexpr int kCellConstant = 0, kCellDynamic = 1, kCellZeroDerivative = 2;
 constexpr int kMaxSortInCell = 64;
+// block size of the persistent LDS-table forms of the neighbour sweeps
+constexpr int kBigBlock = 1024;
+// rho(r^2) tables up to this size are staged in LDS (160 KB per CU on gfx950)
+constexpr size_t kLdsTableMax = 128 * 1024;
+constexpr int kLdsMinAtoms = 32768;
 
 // ---- launch wrappers (eph_kernels.hip) ----
 hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);

@@ -94,11 +94,11 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __
 // fix_eph.cpp:854-862.  The density sweep also compacts the entries with
 // r^2 < rc^2 into an active list, which the w sweep and the force sweep
 // re-read in the same order (they skip exactly the other entries).
+// `rtab` holds the rho(r^2) tables, in global memory or staged in LDS.
 template <int XI, bool FRIC, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
-                                                      WorkView w) {
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i >= a.nlocal) return;
+__device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, const TablesView& t,
+                                               const double4* __restrict__ rtab,
+                                               const StepParams& p, const WorkView& w) {
   const int stride = a.stride;
   const double4 pi = a.pos[i];
   const bool grp = (a.mask[i] & a.groupbit) != 0;
@@ -113,7 +113,7 @@ __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t,
       const double r2 = ex * ex + ey * ey + ez * ez;
       if (r2 < t.rc2) {
         const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-        rho += cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
         w.act[na * stride + i] = j;
         ++na;
       }
@@ -138,7 +138,7 @@ __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t,
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-      const double vr = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+      const double vr = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
       const double pre = alpha * vr / (rho * r2);
       const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
       const double var1 = pre * ev1;
@@ -165,6 +165,50 @@ __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t,
   w.xa[i] = make_double4(x0, x1, x2, alpha);
 }
 
+template <int XI, bool FRIC, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
+                                                      WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI>(i, a, t, t.rho_r2, p, w);
+}
+
+// Persistent form: each block stages the rho(r^2) tables in LDS once and then
+// walks tiles of kBigBlock atoms; the tiles of XCD x form one contiguous range
+// shared by the blocks resident on x.
+__device__ __forceinline__ void xcd_tiles(int ntiles, int* first, int* step) {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int x = b % kXcd, l = b / kXcd;
+  const int nbx = nb / kXcd + (x < nb % kXcd ? 1 : 0);  // blocks on this XCD
+  const int lo = static_cast<int>((static_cast<long>(ntiles) * x) / kXcd);
+  *first = lo + l;
+  *step = nbx;
+  (void)lo;
+}
+
+__device__ __forceinline__ int xcd_tiles_end(int ntiles) {
+  constexpr int kXcd = 8;
+  const int x = blockIdx.x % kXcd;
+  return static_cast<int>((static_cast<long>(ntiles) * (x + 1)) / kXcd);
+}
+
+template <int XI, bool FRIC, bool MULTI>
+__global__ __launch_bounds__(kBigBlock) void k_density_w_lds(AtomsView a, TablesView t,
+                                                             StepParams p, WorkView w) {
+  extern __shared__ double4 s_tab[];
+  const int ntab = t.n_elem * t.n_rho;
+  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  __syncthreads();
+  const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+  int tile, step;
+  xcd_tiles(ntiles, &tile, &step);
+  const int end = xcd_tiles_end(ntiles);
+  for (; tile < end; tile += step) {
+    const int i = tile * kBigBlock + threadIdx.x;
+    if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI>(i, a, t, s_tab, p, w);
+  }
+}
+
 // forward communication of rho, w (fix_eph.cpp:743-744, 870-871) and xi
 // (:863-864) on one process: ghost <- owner
 __global__ __launch_bounds__(kBlock) void k_ghost_rw_xa(AtomsView a, WorkView w) {
@@ -180,10 +224,10 @@ __global__ __launch_bounds__(kBlock) void k_ghost_rw_xa(AtomsView a, WorkView w)
 // application of :892-906.  Both sums share the neighbour gather; each
 // accumulates in list order exactly as its reference loop does.
 template <bool FRIC, bool RAND, bool MULTI>
-__global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
-                                                       StepParams p, WorkView w) {
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i >= a.nlocal) return;
+__device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a, const TablesView& t,
+                                                const double4* __restrict__ rtab,
+                                                const GridView& g, const StepParams& p,
+                                                const WorkView& w) {
   const int stride = a.stride;
   const bool grp = (a.mask[i] & a.groupbit) != 0;
   const double4 rwi = w.rw[i];
@@ -206,8 +250,8 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t
       const double4 xaj = w.xa[j];
       const double alpha_j = xaj.w;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-      const double vr_ji = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
-      const double vr_ij = MULTI ? cubic_at(t.rho_r2 + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
+      const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+      const double vr_ij = MULTI ? cubic_at(rtab + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
       if (FRIC) {
         const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
         const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
@@ -251,6 +295,30 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t
     if (addE) { fx = fx + fex; fy = fy + fey; fz = fz + fez; }
     if (addR) { fx = fx + frx; fy = fy + fry; fz = fz + frz; }
     a.f[i] = fx; a.f[fs + i] = fy; a.f[2 * fs + i] = fz;
+  }
+}
+
+template <bool FRIC, bool RAND, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
+                                                       StepParams p, WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI>(i, a, t, t.rho_r2, g, p, w);
+}
+
+template <bool FRIC, bool RAND, bool MULTI>
+__global__ __launch_bounds__(kBigBlock) void k_prl_forces_lds(AtomsView a, TablesView t, GridView g,
+                                                              StepParams p, WorkView w) {
+  extern __shared__ double4 s_tab[];
+  const int ntab = t.n_elem * t.n_rho;
+  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  __syncthreads();
+  const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+  int tile, step;
+  xcd_tiles(ntiles, &tile, &step);
+  const int end = xcd_tiles_end(ntiles);
+  for (; tile < end; tile += step) {
+    const int i = tile * kBigBlock + threadIdx.x;
+    if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI>(i, a, t, s_tab, g, p, w);
   }
 }
 
@@ -822,11 +890,54 @@ hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int nei
   return hipGetLastError();
 }
 
+// Grid of a persistent LDS-table launch: all blocks that fit at once, a
+// multiple of the 8 XCDs, no more than the tiles need.
+template <typename K>
+static int lds_grid(K kernel, size_t lds_bytes, int ntiles) {
+  (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
+  int occ = 0, dev = 0, cus = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBigBlock, lds_bytes) != hipSuccess ||
+      occ < 1)
+    occ = 1;
+  (void)hipGetDevice(&dev);
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus < 8)
+    cus = 8;
+  int g = occ * cus;
+  const int need = ((ntiles + 7) / 8) * 8;
+  g = g < need ? g : need;
+  g = (g / 8) * 8;
+  return g < 8 ? 8 : g;
+}
+
+static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
+  const size_t bytes = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+  return a.nlocal >= kLdsMinAtoms && bytes <= kLdsTableMax;
+}
+
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
                             const WorkView& w, hipStream_t s) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
+  if (use_lds_tables(a, t)) {
+    const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+    const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+#define EPH_DWL(XI, F, M)                                                                  \
+  k_density_w_lds<XI, F, M><<<lds_grid(k_density_w_lds<XI, F, M>, lds, ntiles), kBigBlock, lds, \
+                              s>>>(a, t, p, w)
+#define EPH_DWL_M(XI, F) if (multi) EPH_DWL(XI, F, true); else EPH_DWL(XI, F, false)
+#define EPH_DWL_F(XI) if (fric) { EPH_DWL_M(XI, true); } else { EPH_DWL_M(XI, false); }
+    switch (p.xi_mode) {
+      case 1: EPH_DWL_F(1); break;
+      case 2: EPH_DWL_F(2); break;
+      default: EPH_DWL_F(0); break;
+    }
+#undef EPH_DWL_F
+#undef EPH_DWL_M
+#undef EPH_DWL
+    return hipGetLastError();
+  }
   const int g = nblk(a.nlocal);
 #define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, w)
 #define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
@@ -853,6 +964,20 @@ hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const Grid
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (!fric && !rnd) return hipSuccess;
+  if (use_lds_tables(a, t)) {
+    const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+    const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+#define EPH_PFL(F, R, M)                                                                 \
+  k_prl_forces_lds<F, R, M><<<lds_grid(k_prl_forces_lds<F, R, M>, lds, ntiles), kBigBlock, lds, \
+                              s>>>(a, t, g, p, w)
+#define EPH_PFL_M(F, R) if (multi) EPH_PFL(F, R, true); else EPH_PFL(F, R, false)
+    if (fric && rnd) { EPH_PFL_M(true, true); }
+    else if (fric) { EPH_PFL_M(true, false); }
+    else { EPH_PFL_M(false, true); }
+#undef EPH_PFL_M
+#undef EPH_PFL
+    return hipGetLastError();
+  }
   const int gr = nblk(a.nlocal);
 #define EPH_PF(F, R, M) k_prl_forces<F, R, M><<<gr, kBlock, 0, s>>>(a, t, g, p, w)
 #define EPH_PF_M(F, R) if (multi) EPH_PF(F, R, true); else EPH_PF(F, R, false)

@@ -548,7 +548,15 @@ int eph_md_rebuild(eph_md* md, const double* x, const double* v) {
 
 int eph_md_attach(eph_md* md, eph_context* ctx) {
   if (!md || !ctx) return EPH_ERR_ARGS;
-  return md->upload(ctx);
+  int rc = md->upload(ctx);
+  if (rc) return rc;
+  // Verlet::setup: pair forces of the initial configuration
+  hipStream_t s = ctx->stream;
+  EPH_HIP(md, launch_ghost_xv(ctx->atoms_view(), ctx->ghost_owner.p, ctx->ghost_shift.p, s));
+  EPH_HIP(md, launch_lj(ctx->atoms_view(), md->lj_nbr.p, md->lj_nn.p, md->lj_stride, md->cfg.lj_eps,
+                        md->cfg.lj_sigma, md->cfg.cut_lj * md->cfg.cut_lj, s));
+  EPH_HIP(md, hipStreamSynchronize(s));
+  return EPH_OK;
 }
 
 int eph_md_set_device_rebuild(eph_md* md, int on) {
