@@ -15,8 +15,9 @@ import numpy as np
 from eph_amd import inputs
 
 
-def workload_files(tmpdir, name, species, cells, a, grid, hot_spot=None):
-    """Write the .beta (and, for a hot spot, the T_e grid file) of a workload."""
+def workload_files(tmpdir, name, species, cells, a, grid, hot_spot=None, steps=1):
+    """Write the .beta (and, for a hot spot, the T_e grid file) of a workload;
+    `steps` is the grid file's initial FDM sub-step count."""
     beta = os.path.join(tmpdir, f"{name}.beta")
     names = inputs.workload_beta(beta, species)
     T_infile = "NULL"
@@ -29,15 +30,16 @@ def workload_files(tmpdir, name, species, cells, a, grid, hot_spot=None):
         c = (idx + 0.5) * d - 0.5 * L
         T = 300.0 + (T_peak - 300.0) * np.exp(-(c * c).sum(1) / (2 * sigma * sigma))
         T_infile = os.path.join(tmpdir, f"{name}.Tin")
-        write_grid_fast(T_infile, n, (0.0, L, 0.0, L, 0.0, L), T, C_e=3e-6, kappa=0.1, rho=1.0)
+        write_grid_fast(T_infile, n, (0.0, L, 0.0, L, 0.0, L), T, C_e=3e-6, kappa=0.1, rho=1.0,
+                        steps=steps)
     return beta, names, T_infile
 
 
-def write_grid_fast(path, n, box, T, C_e, kappa, rho):
+def write_grid_fast(path, n, box, T, C_e, kappa, rho, steps=1):
     idx = np.indices((n, n, n))[::-1].reshape(3, -1).T
     with open(path, "w") as f:
         f.write("# T_e grid\n#\n#\n")
-        f.write(f"{n} {n} {n} 1\n")
+        f.write(f"{n} {n} {n} {steps}\n")
         for a in range(3):
             f.write(f"{float(box[2 * a])!r} {float(box[2 * a + 1])!r}\n")
         f.write("NULL\n")

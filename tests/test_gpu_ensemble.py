@@ -1,0 +1,125 @@
+"""GPU: batched ensembles against separate runs of each replica.
+
+Every replica of an ensemble must follow, bit for bit, the run it would have
+alone on the device path (harness + fix hooks), given the same neighbour
+rebuild steps: the ensemble rebuilds all replicas together and logs the
+steps, and the separate run is forced to rebuild at exactly those steps.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import eph_amd as ea
+from eph_amd import inputs
+from drive import workload_files
+
+pytestmark = pytest.mark.gpu
+
+A = 3.615
+
+
+def _replicas(tmp, specs, species, dt, flags="7", steps=1):
+    """(md keywords, fix args) per replica; all share one .beta file."""
+    out = []
+    for k, sp in enumerate(specs):
+        n = sp["cells"]
+        beta, names, tin = workload_files(tmp, f"r{k}", species, n, A, n, hot_spot=(2000.0, 4.0),
+                                          steps=steps)
+        if k > 0:
+            beta = out[0][1][16]  # the first replica's .beta file
+        args = inputs.fix_args(beta, names, seed=sp["fix_seed"], flags=flags, grid=(n, n, n),
+                               T_infile=tin)
+        kw = dict(cells=n, a=A, ntypes=species, seed=sp["seed"], T=300.0, pka_eV=sp["pka"])
+        out.append((kw, args))
+    return out
+
+
+def _ensemble(reps, species, dt):
+    ens = ea.Ensemble()
+    for kw, args in reps:
+        ens.add(kw, args, species, dt, inputs.BOLTZ, inputs.FTM2V, [inputs.MASS_CU] * species)
+    ens.setup(first_step=1)
+    return ens
+
+
+def _separate(kw, args, species, dt, rebuilt):
+    """the replica alone: device-resident harness run, rebuilt at the logged steps"""
+    kw = dict(kw)
+    cells = kw.pop("cells")
+    md = ea.MD(cells, **kw)
+    inf = md.info()
+    fix = ea.FixEPH(args, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU] * species)
+    fix.init()
+    md.attach(fix)
+    for k, r in enumerate(rebuilt):
+        if r:
+            md.force_rebuild()
+        md.run(fix, 1, first_step=1 + k)
+    return md, fix
+
+
+SPECS = [dict(cells=4, seed=11, fix_seed=101, pka=150.0),
+         dict(cells=5, seed=12, fix_seed=102, pka=200.0),
+         dict(cells=6, seed=13, fix_seed=103, pka=0.0),
+         dict(cells=7, seed=14, fix_seed=104, pka=250.0)]
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_replicas_match_separate_runs(tmp_path, graph):
+    species, dt, nsteps = 2, 5e-4, 30
+    reps = _replicas(str(tmp_path), SPECS, species, dt)
+    ens = _ensemble(reps, species, dt)
+    res = ens.run(nsteps, graph=graph)
+    rebuilt = res["rebuilt"]
+    assert rebuilt.sum() >= 2, rebuilt
+    for r, (kw, args) in enumerate(reps):
+        md, fix = _separate(kw, args, species, dt, rebuilt)
+        n = ens.info(r)["nlocal"]
+        xs, vs, _ = fix.dev_atoms()
+        assert np.array_equal(ens.get(r, "x"), xs[:n]), f"replica {r}: x"
+        assert np.array_equal(ens.get(r, "v"), vs[:n]), f"replica {r}: v"
+        assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), f"replica {r}: T_e"
+        E = ens.get(r, "E")
+        assert E[1] == fix.compute_vector(0), f"replica {r}: Ee {E[1]!r} vs {fix.compute_vector(0)!r}"
+        assert E[0] == fix.last_energy(), f"replica {r}: E_local"
+
+
+def test_ensemble_graph_replay_equals_direct_launches(tmp_path):
+    """the recorded step, replayed, gives the launches' results (several calls)"""
+    species, dt = 2, 5e-4
+    reps = _replicas(str(tmp_path), SPECS[:3], species, dt)
+    a = _ensemble(reps, species, dt)
+    b = _ensemble(reps, species, dt)
+    for n in (7, 1, 12):
+        ra = a.run(n, graph=True)
+        rb = b.run(n, graph=False)
+        assert np.array_equal(ra["rebuilt"], rb["rebuilt"])
+        for r in range(3):
+            for q in ("x", "v", "T", "E"):
+                assert np.array_equal(a.get(r, q), b.get(r, q)), (n, r, q)
+
+
+def test_ensemble_fdm_substeps_differ_per_replica(tmp_path):
+    """replicas with different sub-step counts (grid spacing differs) keep
+    their own count: the ones with fewer sub-steps copy in the extra launches"""
+    species, dt = 1, 1e-3
+    specs = [dict(cells=4, seed=21, fix_seed=201, pka=0.0),
+             dict(cells=6, seed=22, fix_seed=202, pka=0.0)]
+    reps = []
+    for k, sp in enumerate(specs):
+        n = sp["cells"]
+        g = 2 * n if k == 0 else n  # finer grid on replica 0 -> more sub-steps
+        beta, names, tin = workload_files(str(tmp_path), f"s{k}", species, n, A, g, hot_spot=(2000.0, 4.0))
+        if k > 0:
+            beta = reps[0][1][16]
+        args = inputs.fix_args(beta, names, seed=sp["fix_seed"], grid=(g, g, g), T_infile=tin)
+        reps.append((dict(cells=n, a=A, ntypes=species, seed=sp["seed"], T=300.0, pka_eV=0.0), args))
+    ens = _ensemble(reps, species, dt)
+    s0, s1 = ens.info(0)["substeps"], ens.info(1)["substeps"]
+    assert s0 != s1, (s0, s1)
+    res = ens.run(5, graph=True)
+    for r, (kw, args) in enumerate(reps):
+        md, fix = _separate(kw, args, species, dt, res["rebuilt"])
+        assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), r

@@ -7,6 +7,7 @@ that stands in for LAMMPS must produce exact ghosts, complete lists and
 correct pair forces.
 """
 import gzip
+import glob
 import hashlib
 import os
 import shutil
@@ -20,7 +21,7 @@ import minisys
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 G = os.path.join(ROOT, "tests", "golden")
-HEADERS = [os.path.join(ROOT, "include", "eph_amd.h"), os.path.join(ROOT, "include", "eph_md.h")]
+HEADERS = sorted(glob.glob(os.path.join(ROOT, "include", "*.h")))
 
 
 def sha(a):
@@ -91,15 +92,21 @@ def test_fix_command_errors_like_the_reference(tmp_path, mutate, code):
     assert e.value.code == code
 
 
+def _no_gpu():
+    return not os.path.exists("/dev/kfd")
+
+
+@pytest.mark.skipif(not _no_gpu(), reason="a GPU is present")
 def test_valid_fix_needs_a_device_and_says_so(tmp_path):
-    try:
-        import torch
-        if torch.cuda.is_available():
-            pytest.skip("a GPU is present")
-    except ImportError:
-        pass
     with pytest.raises(ea.EphError) as e:
         ea.FixEPH(_args(tmp_path), 1, [0, 0, 0], [10, 10, 10], 1e-3, inputs.BOLTZ, inputs.FTM2V)
+    assert e.value.code == -9
+
+
+@pytest.mark.skipif(not _no_gpu(), reason="a GPU is present")
+def test_ensemble_needs_a_device_and_says_so():
+    with pytest.raises(ea.EphError) as e:
+        ea.Ensemble()
     assert e.value.code == -9
 
 

@@ -95,11 +95,26 @@ void eph_grid::refresh_minmax() {
   minmax_valid = true;
 }
 
-// EPH_FDM::solve (eph_fdm.h:267-400) on the device
-int eph_grid::solve(int* substeps) {
+// sub-step count and inner dt of EPH_FDM::solve (eph_fdm.h:302-314) from the
+// stability extrema
+void eph_grid::substeps(double* r_out, unsigned* steps_out, double* inner_out) const {
   const double steps_d = static_cast<double>(h.steps);
   double inner_dt = h.dt / steps_d;
   const double dtdxdydz = inner_dt * (1.0 / h.dx / h.dx + 1.0 / h.dy / h.dy + 1.0 / h.dz / h.dz);
+  const double r = dtdxdydz / cmin / rmin * kmax;
+  unsigned new_steps = static_cast<unsigned>(h.steps);
+  if (r > 0.4) {
+    inner_dt = 0.4 * inner_dt / r;
+    new_steps = std::max(static_cast<unsigned>(h.dt / inner_dt), 1u);
+    inner_dt = h.dt / new_steps;
+  }
+  *r_out = r;
+  *steps_out = new_steps;
+  *inner_out = inner_dt;
+}
+
+// EPH_FDM::solve (eph_fdm.h:267-400) on the device
+int eph_grid::solve(int* substeps_out) {
   if (any_tdyn) {
     EPH_HIP(this, launch_grid_tdyn_update(v, stream));
     EPH_HIP(this, launch_grid_minmax(v, minmax.p, nullptr, stream));
@@ -110,14 +125,10 @@ int eph_grid::solve(int* substeps) {
   } else if (!minmax_valid) {
     refresh_minmax();
   }
-  const double r = dtdxdydz / cmin / rmin * kmax;
+  double r = 0, inner_dt = 0;
+  unsigned new_steps = 0;
+  substeps(&r, &new_steps, &inner_dt);
   last_r = r;
-  unsigned new_steps = static_cast<unsigned>(h.steps);
-  if (r > 0.4) {
-    inner_dt = 0.4 * inner_dt / r;
-    new_steps = std::max(static_cast<unsigned>(h.dt / inner_dt), 1u);
-    inner_dt = h.dt / new_steps;
-  }
   for (unsigned n = 0; n < new_steps; ++n) {
     EPH_HIP(this, launch_fdm_substep(v, T.p, Tn.p, inner_dt, any_tdyn, stream));
     std::swap(T.p, Tn.p);
@@ -127,7 +138,7 @@ int eph_grid::solve(int* substeps) {
   EPH_HIP(this, launch_fill_double(dT.p, (long)h.ntotal(), 0.0, stream));
   EPH_HIP(this, launch_nonfinite(T.p, (int)h.ntotal(), nonfinite.p, stream));
   last_steps = (int)new_steps;
-  if (substeps) *substeps = (int)new_steps;
+  if (substeps_out) *substeps_out = (int)new_steps;
   return EPH_OK;
 }
 

@@ -90,6 +90,7 @@ struct eph_grid {
   int upload();            // host description -> device (all arrays)
   void refresh_minmax();   // host min/max of the stability estimate
   int solve(int* substeps);
+  void substeps(double* r, unsigned* steps, double* inner_dt) const;
   int insert_points(int n, const double* x, const double* y, const double* z, const double* E);
   int download(int which, std::vector<double>& out);
   int sync_params_from_device();  // C, kappa after temperature-dependent updates

@@ -117,6 +117,11 @@ constexpr int kBigBlock = 1024;
 constexpr size_t kLdsTableMax = 128 * 1024;
 constexpr int kLdsMinAtoms = 32768;
 
+// consecutive atoms [start, start + count) of replica `rep`, count <= kBlock
+struct Chunk {
+  int rep, start, count, pad;
+};
+
 // ---- launch wrappers (eph_kernels.hip) ----
 hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);
 hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
@@ -164,5 +169,28 @@ hipError_t launch_lj(const AtomsView& a, const int* nbr, const int* nn, int stri
 hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* partial, int nblocks,
                             hipStream_t s);
 hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream_t s);
+// batched replicas (one launch per phase for all replicas)
+hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
+                                  const WorkView& w, const Chunk* chunks, int nchunks,
+                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s);
+hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, const GridView* grids,
+                                   const StepParams& p, const WorkView& w, const Chunk* chunks,
+                                   int nchunks, hipStream_t s);
+hipError_t launch_deposit_atoms_batch(const AtomsView& a, const GridView* grids, const int* cell_off,
+                                      const StepParams& p, const WorkView& w, const DepositView& d,
+                                      const Chunk* chunks, int nchunks, hipStream_t s);
+hipError_t launch_energy_reduce_batch(const double* partial, const int* chunk0, const int* nchunks,
+                                      int nrep, double* E_local, double* Ee, hipStream_t s);
+hipError_t launch_fdm_substep_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
+                                    const double* Tin, double* Tout, const double* inner_dt,
+                                    const int* steps, int n, int G, hipStream_t s);
+hipError_t launch_copy_double(const double* src, double* dst, int n, hipStream_t s);
+hipError_t launch_ghost_xv_dev(const AtomsView& a, const int* owner, const double4* shift,
+                               const int* ng, int cap, hipStream_t s);
+hipError_t launch_ghost_rw_xa_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                                  hipStream_t s);
+hipError_t launch_step_advance(uint64_t* step, hipStream_t s);
+hipError_t launch_step_log(const int* flag, const uint64_t* step, const uint64_t* first, int* log,
+                           int nlog, hipStream_t s);
 
 }  // namespace eph

@@ -359,43 +359,50 @@ __global__ void k_fill_double(double* p, long n, double v) {
 // FixEPH::end_of_step (fix_eph.cpp:364-390): per-atom energies of the
 // friction and random forces at end-of-step v, their cell, and block sums
 // of E_local.
+// one atom of the deposit; `coff` is added to its cell (replica offset)
+template <bool FRIC, bool RAND>
+__device__ __forceinline__ void deposit_atom(const int i, const AtomsView& a, const GridView& g,
+                                             const StepParams& p, const WorkView& w,
+                                             const DepositView& d, const int coff, double* eF,
+                                             double* eR) {
+  int c = -1;
+  double v1 = 0.0, v2 = 0.0;
+  if (a.mask[i] & a.groupbit) {
+    const double4 x = a.pos[i];
+    const double4 v = a.vel[i];
+    c = cell_of(g, x.x, x.y, x.z) + coff;
+    const int fs = a.fstride;
+    const double prescale = g.dV * p.dt;
+    if (FRIC) {
+      double dE = 0.0;
+      dE -= w.fE[i] * v.x * p.dt;
+      dE -= w.fE[fs + i] * v.y * p.dt;
+      dE -= w.fE[2 * fs + i] * v.z * p.dt;
+      *eF = dE;
+      v1 = dE / prescale;
+    }
+    if (RAND) {
+      double dE = 0.0;
+      dE -= w.fR[i] * v.x * p.dt;
+      dE -= w.fR[fs + i] * v.y * p.dt;
+      dE -= w.fR[2 * fs + i] * v.z * p.dt;
+      *eR = dE;
+      v2 = dE / prescale;
+    }
+    atomicAdd(&d.count[c], 1);
+  }
+  d.cell[i] = c;
+  d.v1[i] = v1;
+  d.v2[i] = v2;
+}
+
 template <bool FRIC, bool RAND>
 __global__ __launch_bounds__(kBlock) void k_deposit_atoms(AtomsView a, GridView g, StepParams p,
                                                           WorkView w, DepositView d) {
   __shared__ double sh[kBlock];
   const int i = blockIdx.x * kBlock + threadIdx.x;
   double eF = 0.0, eR = 0.0;
-  if (i < a.nlocal) {
-    int c = -1;
-    double v1 = 0.0, v2 = 0.0;
-    if (a.mask[i] & a.groupbit) {
-      const double4 x = a.pos[i];
-      const double4 v = a.vel[i];
-      c = cell_of(g, x.x, x.y, x.z);
-      const int fs = a.fstride;
-      const double prescale = g.dV * p.dt;
-      if (FRIC) {
-        double dE = 0.0;
-        dE -= w.fE[i] * v.x * p.dt;
-        dE -= w.fE[fs + i] * v.y * p.dt;
-        dE -= w.fE[2 * fs + i] * v.z * p.dt;
-        eF = dE;
-        v1 = dE / prescale;
-      }
-      if (RAND) {
-        double dE = 0.0;
-        dE -= w.fR[i] * v.x * p.dt;
-        dE -= w.fR[fs + i] * v.y * p.dt;
-        dE -= w.fR[2 * fs + i] * v.z * p.dt;
-        eR = dE;
-        v2 = dE / prescale;
-      }
-      atomicAdd(&d.count[c], 1);
-    }
-    d.cell[i] = c;
-    d.v1[i] = v1;
-    d.v2[i] = v2;
-  }
+  if (i < a.nlocal) deposit_atom<FRIC, RAND>(i, a, g, p, w, d, 0, &eF, &eR);
   const double sF = block_sum(eF, sh);
   const double sR = block_sum(eR, sh);
   if (threadIdx.x == 0) {
@@ -601,9 +608,8 @@ __global__ __launch_bounds__(kBlock) void k_cell_sum_huge(GridView g, DepositVie
 }
 
 // E_local = sum of friction then random energies; Ee += E_local
-__global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, int nb,
-                                                        double* E_local, double* Ee) {
-  __shared__ double sh[1024];
+__device__ __forceinline__ void energy_reduce(const double* partial, int nb, double* E_local,
+                                              double* Ee, double* sh) {
   double sF = 0.0, sR = 0.0;
   for (int b = threadIdx.x; b < nb; b += 1024) {
     sF += partial[2 * b];
@@ -616,6 +622,12 @@ __global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, i
     *E_local = E;
     *Ee += E;
   }
+}
+
+__global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, int nb,
+                                                        double* E_local, double* Ee) {
+  __shared__ double sh[1024];
+  energy_reduce(partial, nb, E_local, Ee, sh);
 }
 
 // ------------------------------------------------------------- grid
@@ -680,11 +692,10 @@ __device__ __forceinline__ double linear_rev(const GridView& g, double y) {
 // one explicit sub-step of EPH_FDM::solve (eph_fdm.h:315-395): the 7-point
 // variable-kappa stencil on Tin, the update and the clamp, into Tout
 template <bool TDYN>
-__global__ __launch_bounds__(kBlock) void k_fdm_substep(GridView g, const double* __restrict__ Tin,
-                                                        double* __restrict__ Tout, double inner_dt) {
-  const int r = blockIdx.x * kBlock + threadIdx.x;
+__device__ __forceinline__ void fdm_cell(const GridView& g, const double* __restrict__ Tin,
+                                         double* __restrict__ Tout, const int r,
+                                         const double inner_dt) {
   const int nx = g.nx, ny = g.ny, nz = g.nz;
-  if (r >= nx * ny * nz) return;
   const int i = r % nx, j = (r / nx) % ny, k = r / (nx * ny);
   const int fr = g.flag[r];
   double ddT = 0.0;
@@ -727,6 +738,14 @@ __global__ __launch_bounds__(kBlock) void k_fdm_substep(GridView g, const double
   }
   if (T < 0.0) T = 0.0;
   Tout[r] = T;
+}
+
+template <bool TDYN>
+__global__ __launch_bounds__(kBlock) void k_fdm_substep(GridView g, const double* __restrict__ Tin,
+                                                        double* __restrict__ Tout, double inner_dt) {
+  const int r = blockIdx.x * kBlock + threadIdx.x;
+  if (r >= g.nx * g.ny * g.nz) return;
+  fdm_cell<TDYN>(g, Tin, Tout, r, inner_dt);
 }
 
 __global__ __launch_bounds__(kBlock) void k_grid_lookup(GridView g, int n, const double* x,
@@ -833,6 +852,129 @@ __global__ __launch_bounds__(kBlock) void k_max_disp2(AtomsView a, const double4
   if (threadIdx.x == 0) partial[blockIdx.x] = sh[0];
 }
 
+// ============================================================ batched replicas
+// Several independent systems share one launch per phase.  A chunk is up to
+// kBlock consecutive atoms of one replica and block b handles chunk b, so the
+// replica's noise seed and T_e grid are uniform in a block and its energy
+// partials line up with the blocks of a single-system launch: every replica
+// gets the same bits it would get alone.
+// the step number is read from device memory so a recorded step replays
+template <int XI, bool FRIC, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_density_w_batch(AtomsView a, TablesView t, StepParams p,
+                                                            WorkView w,
+                                                            const Chunk* __restrict__ chunks,
+                                                            const uint32_t* __restrict__ seeds,
+                                                            const uint64_t* __restrict__ step) {
+  const Chunk c = chunks[blockIdx.x];
+  if (static_cast<int>(threadIdx.x) >= c.count) return;
+  StepParams pr = p;
+  pr.seed = seeds[c.rep];
+  pr.step = *step;
+  density_w_atom<XI, FRIC, MULTI>(c.start + threadIdx.x, a, t, t.rho_r2, pr, w);
+}
+
+// start of a recorded step: step += 1
+__global__ void k_step_advance(uint64_t* step) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *step += 1;
+}
+
+// log[step - first] = *flag (which steps rebuilt)
+__global__ void k_step_log(const int* __restrict__ flag, const uint64_t* __restrict__ step,
+                           const uint64_t* __restrict__ first, int* log, int nlog) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  const long k = static_cast<long>(*step - *first);
+  if (k >= 0 && k < nlog) log[k] = *flag;
+}
+
+template <bool FRIC, bool RAND, bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, TablesView t,
+                                                             const GridView* __restrict__ grids,
+                                                             StepParams p, WorkView w,
+                                                             const Chunk* __restrict__ chunks) {
+  const Chunk c = chunks[blockIdx.x];
+  if (static_cast<int>(threadIdx.x) >= c.count) return;
+  prl_forces_atom<FRIC, RAND, MULTI>(c.start + threadIdx.x, a, t, t.rho_r2, grids[c.rep], p, w);
+}
+
+template <bool FRIC, bool RAND>
+__global__ __launch_bounds__(kBlock) void k_deposit_atoms_batch(AtomsView a,
+                                                                const GridView* __restrict__ grids,
+                                                                const int* __restrict__ cell_off,
+                                                                StepParams p, WorkView w,
+                                                                DepositView d,
+                                                                const Chunk* __restrict__ chunks) {
+  __shared__ double sh[kBlock];
+  const Chunk c = chunks[blockIdx.x];
+  double eF = 0.0, eR = 0.0;
+  if (static_cast<int>(threadIdx.x) < c.count)
+    deposit_atom<FRIC, RAND>(c.start + threadIdx.x, a, grids[c.rep], p, w, d, cell_off[c.rep], &eF,
+                             &eR);
+  const double sF = block_sum(eF, sh);
+  const double sR = block_sum(eR, sh);
+  if (threadIdx.x == 0) {
+    d.partial[2 * blockIdx.x] = sF;
+    d.partial[2 * blockIdx.x + 1] = sR;
+  }
+}
+
+__global__ __launch_bounds__(1024) void k_energy_reduce_batch(const double* partial,
+                                                              const int* __restrict__ chunk0,
+                                                              const int* __restrict__ nchunks,
+                                                              double* E_local, double* Ee) {
+  __shared__ double sh[1024];
+  const int r = blockIdx.x;
+  energy_reduce(partial + 2 * chunk0[r], nchunks[r], E_local + r, Ee + r, sh);
+}
+
+// one sub-step over all replicas' cells; a replica whose sub-step count is
+// below n copies its cells, so the two buffers stay in step for everyone
+__global__ __launch_bounds__(kBlock) void k_fdm_substep_batch(const GridView* __restrict__ grids,
+                                                              const int* __restrict__ cell_rep,
+                                                              const int* __restrict__ cell_off,
+                                                              const double* __restrict__ Tin,
+                                                              double* __restrict__ Tout,
+                                                              const double* __restrict__ inner_dt,
+                                                              const int* __restrict__ steps, int n,
+                                                              int G) {
+  const int gc = blockIdx.x * kBlock + threadIdx.x;
+  if (gc >= G) return;
+  const int r = cell_rep[gc];
+  const int off = cell_off[r];
+  if (n >= steps[r]) {
+    Tout[gc] = Tin[gc];
+    return;
+  }
+  fdm_cell<false>(grids[r], Tin + off, Tout + off, gc - off, inner_dt[r]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy_double(const double* __restrict__ src,
+                                                        double* __restrict__ dst, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i < n) dst[i] = src[i];
+}
+
+// ghost updates with the ghost count read on the device (graph-capturable)
+__global__ __launch_bounds__(kBlock) void k_ghost_xv_dev(AtomsView a, const int* __restrict__ owner,
+                                                         const double4* __restrict__ shift,
+                                                         const int* __restrict__ ng) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= *ng) return;
+  const int o = owner[g];
+  const double4 p = a.pos[o];
+  const double4 s = shift[g];
+  a.pos[a.nlocal + g] = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, p.w);
+  a.vel[a.nlocal + g] = a.vel[o];
+}
+
+__global__ __launch_bounds__(kBlock) void k_ghost_rw_xa_dev(AtomsView a, WorkView w,
+                                                            const int* __restrict__ ng) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= *ng) return;
+  const int o = w.ghost_owner[g];
+  w.rw[a.nlocal + g] = w.rw[o];
+  w.xa[a.nlocal + g] = w.xa[o];
+}
+
 // ---------------------------------------------------- host layout staging
 // LAMMPS [n][3] rows <-> device double4 (keeping .w) and [3][stride] arrays
 __global__ __launch_bounds__(kBlock) void k_rows_into_d4(const double* __restrict__ rows, int n,
@@ -910,9 +1052,16 @@ static int lds_grid(K kernel, size_t lds_bytes, int ntiles) {
   return g < 8 ? 8 : g;
 }
 
+// The tables go to LDS only when the device reports room for them in one
+// workgroup (gfx950: 160 KB); otherwise the sweeps read them from global memory.
 static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
   const size_t bytes = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-  return a.nlocal >= kLdsMinAtoms && bytes <= kLdsTableMax;
+  if (a.nlocal < kLdsMinAtoms || bytes > kLdsTableMax) return false;
+  int dev = 0, lds_max = 0;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return false;
+  return bytes <= static_cast<size_t>(lds_max);
 }
 
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
@@ -1153,6 +1302,100 @@ hipError_t launch_soa_into_rows(const double* soa, int n, int stride, double* ro
 hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_d4_component<<<nblk(n), kBlock, 0, s>>>(in, n, comp, out);
+  return hipGetLastError();
+}
+
+
+// ---------------------------------------------------------- batched wrappers
+hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
+                                  const WorkView& w, const Chunk* chunks, int nchunks,
+                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s) {
+  if (nchunks == 0) return hipSuccess;
+  const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
+#define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, 0, s>>>(a, t, p, w, chunks, seeds, step)
+#define EPH_DWB_M(XI, F) if (multi) EPH_DWB(XI, F, true); else EPH_DWB(XI, F, false)
+#define EPH_DWB_F(XI) if (fric) { EPH_DWB_M(XI, true); } else { EPH_DWB_M(XI, false); }
+  switch (p.xi_mode) {
+    case 1: EPH_DWB_F(1); break;
+    case 2: EPH_DWB_F(2); break;
+    default: EPH_DWB_F(0); break;
+  }
+#undef EPH_DWB_F
+#undef EPH_DWB_M
+#undef EPH_DWB
+  return hipGetLastError();
+}
+
+hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, const GridView* grids,
+                                   const StepParams& p, const WorkView& w, const Chunk* chunks,
+                                   int nchunks, hipStream_t s) {
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
+  if (nchunks == 0 || (!fric && !rnd)) return hipSuccess;
+#define EPH_PFB(F, R, M) k_prl_forces_batch<F, R, M><<<nchunks, kBlock, 0, s>>>(a, t, grids, p, w, chunks)
+#define EPH_PFB_M(F, R) if (multi) EPH_PFB(F, R, true); else EPH_PFB(F, R, false)
+  if (fric && rnd) { EPH_PFB_M(true, true); }
+  else if (fric) { EPH_PFB_M(true, false); }
+  else { EPH_PFB_M(false, true); }
+#undef EPH_PFB_M
+#undef EPH_PFB
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit_atoms_batch(const AtomsView& a, const GridView* grids, const int* cell_off,
+                                      const StepParams& p, const WorkView& w, const DepositView& d,
+                                      const Chunk* chunks, int nchunks, hipStream_t s) {
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02;
+  if (nchunks == 0) return hipSuccess;
+  if (fric && rnd) k_deposit_atoms_batch<true, true><<<nchunks, kBlock, 0, s>>>(a, grids, cell_off, p, w, d, chunks);
+  else if (fric) k_deposit_atoms_batch<true, false><<<nchunks, kBlock, 0, s>>>(a, grids, cell_off, p, w, d, chunks);
+  else if (rnd) k_deposit_atoms_batch<false, true><<<nchunks, kBlock, 0, s>>>(a, grids, cell_off, p, w, d, chunks);
+  else k_deposit_atoms_batch<false, false><<<nchunks, kBlock, 0, s>>>(a, grids, cell_off, p, w, d, chunks);
+  return hipGetLastError();
+}
+
+hipError_t launch_energy_reduce_batch(const double* partial, const int* chunk0, const int* nchunks,
+                                      int nrep, double* E_local, double* Ee, hipStream_t s) {
+  if (nrep == 0) return hipSuccess;
+  k_energy_reduce_batch<<<nrep, 1024, 0, s>>>(partial, chunk0, nchunks, E_local, Ee);
+  return hipGetLastError();
+}
+
+hipError_t launch_fdm_substep_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
+                                    const double* Tin, double* Tout, const double* inner_dt,
+                                    const int* steps, int n, int G, hipStream_t s) {
+  if (G == 0) return hipSuccess;
+  k_fdm_substep_batch<<<nblk(G), kBlock, 0, s>>>(grids, cell_rep, cell_off, Tin, Tout, inner_dt, steps, n, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_copy_double(const double* src, double* dst, int n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_copy_double<<<nblk(n), kBlock, 0, s>>>(src, dst, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_xv_dev(const AtomsView& a, const int* owner, const double4* shift,
+                               const int* ng, int cap, hipStream_t s) {
+  if (cap == 0) return hipSuccess;
+  k_ghost_xv_dev<<<nblk(cap), kBlock, 0, s>>>(a, owner, shift, ng);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_rw_xa_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                                  hipStream_t s) {
+  if (cap == 0) return hipSuccess;
+  k_ghost_rw_xa_dev<<<nblk(cap), kBlock, 0, s>>>(a, w, ng);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_advance(uint64_t* step, hipStream_t s) {
+  k_step_advance<<<1, 64, 0, s>>>(step);
+  return hipGetLastError();
+}
+
+hipError_t launch_step_log(const int* flag, const uint64_t* step, const uint64_t* first, int* log,
+                           int nlog, hipStream_t s) {
+  k_step_log<<<1, 64, 0, s>>>(flag, step, first, log, nlog);
   return hipGetLastError();
 }
 

@@ -7,6 +7,7 @@ is missing or no HIP device is present, the calls raise.
     FixEPH  -- the fix (FixEPH hooks, fix_eph.cpp), host-array or device-resident
     Grid    -- the electronic T_e grid (EPH_FDM, eph_fdm.h), also standalone
     MD      -- the minimal MD harness standing in for LAMMPS (include/eph_md.h)
+    Ensemble -- many replicas advanced together (include/eph_ensemble.h)
 """
 import ctypes as C
 import os
@@ -112,6 +113,16 @@ _SIG = {
     "eph_md_set_device_rebuild": (C.c_int, [vp, C.c_int]),
     "eph_md_force_rebuild": (C.c_int, [vp]),
     "eph_md_pull": (C.c_int, [vp, vp]),
+    "eph_ensemble_create": (C.c_int, [C.POINTER(vp), C.c_int]),
+    "eph_ensemble_destroy": (None, [vp]),
+    "eph_ensemble_last_error": (C.c_char_p, [vp]),
+    "eph_ensemble_add": (C.c_int, [vp, C.POINTER(MDConfig), C.c_int, C.POINTER(C.c_char_p), C.c_int,
+                                   C.c_double, C.c_double, C.c_double, dp]),
+    "eph_ensemble_setup": (C.c_int, [vp, C.c_longlong]),
+    "eph_ensemble_run": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
+    "eph_ensemble_size": (C.c_int, [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "eph_ensemble_info": (C.c_int, [vp, C.c_int, ip, ip, ip]),
+    "eph_ensemble_get": (C.c_int, [vp, C.c_int, C.c_int, dp]),
 }
 
 _lib = None
@@ -520,6 +531,74 @@ class MD:
         inf = self.info()
         fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
         return dict(fix_ms=t[0], total_ms=t[1], pair_ms=t[2], rebuild_ms=t[3], rebuilds=nreb.value)
+
+
+class Ensemble:
+    """Replicas of (harness system, fix) advanced together on one GPU
+    (include/eph_ensemble.h).  Each replica keeps its own seed, T_e grid and
+    energy tally; results equal separate runs rebuilt at the same steps."""
+
+    def __init__(self, device=0):
+        h = vp()
+        rc = lib().eph_ensemble_create(C.byref(h), device)
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_ensemble_last_error(None).decode())
+        self.h = h
+        self.nrep = 0
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().eph_ensemble_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_ensemble_last_error(self.h).decode())
+
+    def add(self, md_kwargs, args, ntypes, dt, boltz, ftm2v, masses):
+        """md_kwargs: MD(...) keywords (cells=..., seed=..., pka_eV=...);
+        args: the fix command words (box taken from the replica's system)."""
+        kw = dict(a=3.615, ntypes=1, seed=1, T=300.0, disp_sigma=0.0, pka_eV=0.0, mass=63.546,
+                  cut_fix=4.5, cut_lj=5.5, skin=2.0, lj_eps=0.4, lj_sigma=2.3)
+        kw.update(md_kwargs)
+        cfg = MDConfig(kw["cells"], kw["a"], kw["ntypes"], kw["seed"] & 0xFFFFFFFF, kw["T"],
+                       kw["disp_sigma"], kw["pka_eV"], kw["mass"], kw["cut_fix"], kw["cut_lj"],
+                       kw["skin"], kw["lj_eps"], kw["lj_sigma"])
+        arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+        m = f64(masses)
+        self._check(lib().eph_ensemble_add(self.h, C.byref(cfg), len(args), arr, ntypes, dt, boltz,
+                                           ftm2v, _d(m)))
+        self.nrep += 1
+
+    def setup(self, first_step=1):
+        self._check(lib().eph_ensemble_setup(self.h, first_step))
+
+    def run(self, nsteps, graph=True, log=True):
+        ms = C.c_double()
+        rebuilt = np.zeros(max(nsteps, 1), np.int32)
+        self._check(lib().eph_ensemble_run(self.h, nsteps, 1 if graph else 0, C.byref(ms),
+                                           _i(rebuilt) if log else None))
+        return dict(ms=ms.value, rebuilt=rebuilt[:nsteps] if log else None)
+
+    def size(self):
+        n, na, nc = C.c_int(), C.c_longlong(), C.c_longlong()
+        self._check(lib().eph_ensemble_size(self.h, C.byref(n), C.byref(na), C.byref(nc)))
+        return dict(replicas=n.value, atoms=na.value, cells=nc.value)
+
+    def info(self, r):
+        nl, nc, ns = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().eph_ensemble_info(self.h, r, C.byref(nl), C.byref(nc), C.byref(ns)))
+        return dict(nlocal=nl.value, cells=nc.value, substeps=ns.value)
+
+    def get(self, r, which):
+        """which: 'x', 'v', 'f' ([nlocal][3]), 'T' ([cells]), 'E' ((E_local, Ee))"""
+        k = {"x": 0, "v": 1, "f": 2, "T": 3, "E": 4}[which]
+        inf = self.info(r)
+        out = np.zeros((inf["nlocal"], 3)) if k < 3 else np.zeros(inf["cells"] if k == 3 else 2)
+        self._check(lib().eph_ensemble_get(self.h, r, k, _d(out)))
+        return out
 
 
 def exported_symbols(header_paths):
