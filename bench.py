@@ -82,6 +82,99 @@ def cpu_baseline(args_fix, species, snap, md, dt, nsteps):
     return snap.nlocal * nsteps / t_fix, t_fix
 
 
+ENS_REPLICAS, ENS_STEPS, ENS_DT = 64, 300, 2.5e-4
+
+
+def write_uniform_grid(path, n, L, steps):
+    import numpy as np
+    idx = np.indices((n, n, n))[::-1].reshape(3, -1).T
+    with open(path, "w") as f:
+        f.write("# T_e grid: uniform 300 K\n#\n#\n")
+        f.write(f"{n} {n} {n} {steps}\n")
+        for _ in range(3):
+            f.write(f"0.0 {L!r}\n")
+        f.write("NULL\n")
+        f.write("\n".join(f"{i} {j} {k} 300.0 0.0 1.0 3e-06 0.1 1 0" for i, j, k in idx.tolist()))
+        f.write("\n")
+
+
+def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
+    """Batched ensemble: nrep replicas of n^3 fcc cells, n ~ logU[10, 25]
+    (seeded), one 1 keV PKA each, own seed and n^3 T_e grid (19 sub-steps),
+    advanced together; against the same replicas run one after another."""
+    import numpy as np
+    import eph_amd as ea
+    from eph_amd import inputs
+    a = 3.615
+    rng = np.random.default_rng(3)
+    ns = np.rint(np.exp(rng.uniform(np.log(10.0), np.log(25.0), nrep))).astype(int)
+    beta = os.path.join(tmp, "ens.beta")
+    names = inputs.workload_beta(beta, 1)
+    reps = []
+    for r, n in enumerate(ns.tolist()):
+        tin = os.path.join(tmp, f"ens{r}.T")
+        write_uniform_grid(tin, n, n * a, 19)
+        fargs = inputs.fix_args(beta, names, seed=7000 + r, flags="7", grid=(n, n, n),
+                                T_infile=tin, C_e=3e-6, kappa=0.1)
+        kw = dict(cells=n, a=a, ntypes=1, seed=900 + r, T=300.0, pka_eV=1000.0)
+        reps.append((kw, fargs))
+    t0 = time.time()
+    ens = ea.Ensemble()
+    for kw, fargs in reps:
+        ens.add(kw, fargs, 1, ENS_DT, inputs.BOLTZ, inputs.FTM2V, [inputs.MASS_CU])
+    ens.setup(first_step=1)
+    size = ens.size()
+    sub = ens.info(0)["substeps"]
+    log(f"ensemble: {nrep} replicas, {size['atoms']} atoms, {size['cells']} cells, "
+        f"{sub} sub-steps ({time.time() - t0:.1f} s)")
+    # launch mode: recorded graph or direct launches, whichever is faster on
+    # a short trial (both give the same bits)
+    trial = {}
+    for g in (True, False):
+        trial[g] = ens.run(max(warmup, 2), graph=g, log=False)["ms"]
+    use_graph = trial[True] <= trial[False]
+    w0 = time.perf_counter()
+    res = ens.run(nsteps, graph=use_graph, log=True)
+    wall = time.perf_counter() - w0
+    out = {
+        "workload": f"ens{nrep}_fcc_logU10-25_pka1keV",
+        "replicas": nrep, "atoms": size["atoms"], "Te_cells": size["cells"],
+        "steps": nsteps, "dt_ps": ENS_DT, "substeps": sub, "pka_eV": 1000.0,
+        "launch": "graph" if use_graph else "direct",
+        "trial_ms_per_step": {"graph": trial[True] / max(warmup, 2),
+                              "direct": trial[False] / max(warmup, 2)},
+        "ms_per_step": res["ms"] / nsteps,
+        "wall_ms_per_step": wall * 1e3 / nsteps,
+        "rebuild_steps": int(res["rebuilt"].sum()),
+        "replica_steps_per_s": nrep * nsteps / wall,
+        "atom_steps_per_s": size["atoms"] * nsteps / wall,
+    }
+    ens.close()
+    if sequential:
+        tot = 0.0
+        t1 = time.time()
+        for kw, fargs in reps:
+            kw = dict(kw)
+            n = kw.pop("cells")
+            md = ea.MD(n, **kw)
+            inf = md.info()
+            fx = ea.FixEPH(fargs, 1, inf["boxlo"], inf["boxhi"], ENS_DT, inputs.BOLTZ,
+                           inputs.FTM2V, masses=[inputs.MASS_CU])
+            fx.init()
+            md.attach(fx)
+            md.run(fx, max(warmup, 2), first_step=1)
+            w0 = time.perf_counter()
+            md.run(fx, nsteps, first_step=1 + max(warmup, 2))
+            tot += time.perf_counter() - w0
+            md.close()
+            fx.close()
+        seq = nrep * nsteps / tot
+        out["sequential_replica_steps_per_s"] = seq
+        out["speedup_vs_sequential"] = out["replica_steps_per_s"] / seq
+        log(f"sequential: {tot:.2f} s for {nrep} x {nsteps} steps ({time.time() - t1:.1f} s)")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,8 +184,13 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="also run a plain device copy of MIB MiB (torch) three times, to scale "
-                         "the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
+                    help="also run a plain device copy (k_copy_probe) of MIB MiB three times, to "
+                         "scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run")
+    ap.add_argument("--ensemble-replicas", type=int, default=ENS_REPLICAS,
+                    help="replicas of the batched-ensemble line (0 = skip it)")
+    ap.add_argument("--ensemble-steps", type=int, default=ENS_STEPS)
+    ap.add_argument("--no-sequential", action="store_true",
+                    help="skip the one-replica-at-a-time comparison of the ensemble line")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -225,6 +323,14 @@ def main():
         ms = ea.copy_probe(nbytes, 3)
         out["calibration_copy"] = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": ms,
                                    "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
+    if args.ensemble_replicas > 0:
+        fix.close()
+        md.close()
+        try:
+            out["ensemble"] = ensemble_bench(tmp, args.ensemble_replicas, args.ensemble_steps,
+                                             args.warmup, not args.no_sequential)
+        except ea.EphError as e:
+            out["ensemble"] = {"error": str(e)}
     print(json.dumps(out), flush=True)
 
 

@@ -40,7 +40,8 @@ const char* eph_ensemble_last_error(const eph_ensemble* e);
 /* Add a replica: the harness system of cfg and a fix built from the fix
  * command words arg (as eph_create takes them) on that system's box.  All
  * replicas must share the .beta file, flags, model, dt and masses; each has
- * its own seed and T_e grid (constant C_e and kappa). */
+ * its own seed and T_e grid (constant C_e and kappa).  masses has ntypes + 1
+ * entries in the layout of eph_set_masses (LAMMPS atom->mass, [0] unused). */
 int eph_ensemble_add(eph_ensemble* e, const eph_md_config* cfg, int narg, const char* const* arg,
                      int ntypes, double dt, double boltz, double ftm2v, const double* masses);
 

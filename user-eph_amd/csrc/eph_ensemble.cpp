@@ -385,7 +385,7 @@ int eph_ensemble_add(eph_ensemble* e, const eph_md_config* cfg, int narg, const 
     if (cfg->cut_fix != c.cut_fix || cfg->cut_lj != c.cut_lj || cfg->skin != c.skin ||
         cfg->lj_eps != c.lj_eps || cfg->lj_sigma != c.lj_sigma || ntypes != e->ntypes || dt != e->dt)
       return e->fail(EPH_ERR_ARGS, "replicas must share cutoffs, skin, pair, types and dt");
-    for (int t = 0; t < ntypes; ++t)
+    for (int t = 1; t <= ntypes; ++t)
       if (masses[t] != e->masses[t]) return e->fail(EPH_ERR_ARGS, "replicas must share masses");
   }
   eph_ensemble::Replica r;
@@ -413,7 +413,7 @@ int eph_ensemble_add(eph_ensemble* e, const eph_md_config* cfg, int narg, const 
     e->args0.assign(arg, arg + narg);
     e->ntypes = ntypes;
     e->dt = dt; e->boltz = boltz; e->ftm2v = ftm2v;
-    e->masses.assign(masses, masses + ntypes);
+    e->masses.assign(masses, masses + ntypes + 1);
   } else {
     const eph_context* f0 = e->reps[0].fix;
     bool same = narg == (int)e->args0.size();

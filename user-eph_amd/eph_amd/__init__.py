@@ -567,7 +567,9 @@ class Ensemble:
                        kw["disp_sigma"], kw["pka_eV"], kw["mass"], kw["cut_fix"], kw["cut_lj"],
                        kw["skin"], kw["lj_eps"], kw["lj_sigma"])
         arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
-        m = f64(masses)
+        m = f64(np.concatenate([[0.0], np.asarray(masses, dtype=np.float64)]))
+        if len(m) != ntypes + 1:
+            raise ValueError("one mass per atom type")
         self._check(lib().eph_ensemble_add(self.h, C.byref(cfg), len(args), arr, ntypes, dt, boltz,
                                            ftm2v, _d(m)))
         self.nrep += 1
