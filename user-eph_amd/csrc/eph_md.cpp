@@ -172,7 +172,12 @@ int eph_md::upload(eph_context* ctx) {
     return fail(rc, ctx->err);
   if ((rc = ctx->set_ghost_map(nghost, owner.data(), shift.data()))) return fail(rc, ctx->err);
   if ((rc = ctx->set_neighbors(nlocal, fix.off.data(), fix.idx.data(), ~0))) return fail(rc, ctx->err);
-  // lj list in the context's ELL stride
+  // lj list in the context's ELL stride (indices checked like the fix list's)
+  {
+    const long nall = (long)nlocal + nghost;
+    for (int j : lj.idx)
+      if (j < 0 || j >= nall) return fail(EPH_ERR_STATE, "pair list index out of range");
+  }
   mx = cap_l;
   lj_stride = ctx->stride;
   hipStream_t s = ctx->stream;
