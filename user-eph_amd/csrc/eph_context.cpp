@@ -215,7 +215,7 @@ StepParams eph_context::step_params() {
 WorkView eph_context::work_view() {
   WorkView w;
   w.nbr = nbr.p; w.nnbr = nnbr.p; w.act = act.p; w.nact = nact.p;
-  w.rw = rw.p; w.xa = xa.p; w.fE = fE.p; w.fR = fR.p; w.flags_out = flags_dev.p;
+  w.rec = rec.p; w.pv = pv.p; w.fE = fE.p; w.fR = fR.p; w.flags_out = flags_dev.p;
   w.ghost_owner = ghost_owner.p; w.ghost_shift = ghost_shift.p;
   return w;
 }
@@ -225,8 +225,8 @@ int eph_context::reserve_atoms(int nl, int ng) {
   const int st = ((nl + 63) / 64) * 64 + 64;
   EPH_HIP(this, pos.reserve(nall));
   EPH_HIP(this, vel.reserve(nall));
-  EPH_HIP(this, rw.reserve(nall));
-  EPH_HIP(this, xa.reserve(nall));
+  EPH_HIP(this, rec.reserve(nall));
+  EPH_HIP(this, pv.reserve(nall));
   EPH_HIP(this, type.reserve(nall));
   EPH_HIP(this, mask.reserve(nall));
   EPH_HIP(this, tag.reserve(nall));
@@ -397,12 +397,15 @@ int eph_context::dev_post_force() {
   const AtomsView a = atoms_view();
   const WorkView w = work_view();
   hipEvent_t e;
+  timed_begin("pack_pv", &e);
+  EPH_HIP(this, launch_pack_pv(a, w, stream));
+  timed_end("pack_pv", e);
   timed_begin("density_w", &e);
   EPH_HIP(this, launch_density_w(a, tabs, p, w, stream));
   timed_end("density_w", e);
-  timed_begin("ghost_rw_xa", &e);
-  EPH_HIP(this, launch_ghost_rw_xa(a, w, stream));
-  timed_end("ghost_rw_xa", e);
+  timed_begin("ghost_rec", &e);
+  EPH_HIP(this, launch_ghost_rec(a, w, stream));
+  timed_end("ghost_rec", e);
   if (model == 4) {
     timed_begin("prl_forces", &e);
     EPH_HIP(this, launch_prl_forces(a, tabs, grid->v, p, w, stream));
@@ -679,9 +682,9 @@ static int upload_rows(eph_context* c, const double* rows, int n, double4* dst) 
   return EPH_OK;
 }
 
-static int download_rows(eph_context* c, const double4* src, int n, double* rows) {
+static int download_rows(eph_context* c, const double4* src, int n, double* rows, int stride4 = 1) {
   if (n == 0) return EPH_OK;
-  EPH_HIP(c, launch_d4_into_rows(src, n, c->stage.p, c->stream));
+  EPH_HIP(c, launch_d4_into_rows(src, n, c->stage.p, c->stream, stride4));
   EPH_HIP(c, hipMemcpyAsync(rows, c->stage.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
   return EPH_OK;
 }
@@ -802,12 +805,12 @@ int eph_get_state(eph_context* ctx, int which, double* out) {
   const int nall = ctx->nlocal + ctx->nghost;
   hipStream_t s = ctx->stream;
   switch (which) {
-    case 0: EPH_HIP(ctx, launch_d4_component(ctx->rw.p, nall, 3, ctx->stage.p, s));
+    case 0: EPH_HIP(ctx, launch_d4_component(ctx->rec_field(1), nall, 3, ctx->stage.p, s, 4));
       EPH_HIP(ctx, hipMemcpyAsync(out, ctx->stage.p, nall * sizeof(double), hipMemcpyDeviceToHost, s)); break;
-    case 5: EPH_HIP(ctx, launch_d4_component(ctx->xa.p, nall, 3, ctx->stage.p, s));
+    case 5: EPH_HIP(ctx, launch_d4_component(ctx->rec_field(2), nall, 3, ctx->stage.p, s, 4));
       EPH_HIP(ctx, hipMemcpyAsync(out, ctx->stage.p, nall * sizeof(double), hipMemcpyDeviceToHost, s)); break;
-    case 1: { int rc = download_rows(ctx, ctx->rw.p, nall, out); if (rc) return rc; break; }
-    case 2: { int rc = download_rows(ctx, ctx->xa.p, nall, out); if (rc) return rc; break; }
+    case 1: { int rc = download_rows(ctx, ctx->rec_field(1), nall, out, 4); if (rc) return rc; break; }
+    case 2: { int rc = download_rows(ctx, ctx->rec_field(2), nall, out, 4); if (rc) return rc; break; }
     case 3: case 4: {
       const double* src = which == 3 ? ctx->fE.p : ctx->fR.p;
       EPH_HIP(ctx, launch_soa_into_rows(src, ctx->nlocal, ctx->fstride, ctx->stage.p, s));

@@ -113,7 +113,11 @@ struct eph_context {
   // atoms
   int nlocal = 0, nghost = 0, stride = 0, fstride = 0, maxnbr = 0;
   bool have_atoms = false, have_list = false, have_forces = false;
-  eph::DevArray<double4> pos, vel, rw, xa, ghost_shift, stage4;
+  eph::DevArray<double4> pos, vel, ghost_shift, stage4;
+  eph::DevArray<eph::ForceRec> rec;
+  eph::DevArray<eph::PosVel> pv;
+  // field f of every ForceRec (0 position, 1 (w, rho), 2 (xi, alpha)), stride 4 double4
+  const double4* rec_field(int f) const { return reinterpret_cast<const double4*>(rec.p) + f; }
   eph::DevArray<double> f, fE, fR, xi_in, stage, mass, scal;  // scal: [0] E_local [1] Ee
   eph::DevArray<int> type, mask, tag, ghost_owner, nbr, nnbr, act, nact, csr_off, csr_idx,
       flags_dev;

@@ -3,8 +3,9 @@
 // Layout in HBM (FP64 throughout):
 //   pos[nall]  double4 (x, y, z, element index)   gathered per neighbour
 //   vel[nall]  double4 (vx, vy, vz, 0)             gathered in the w sweep
-//   rw[nall]   double4 (w_x, w_y, w_z, rho)        gathered in the force sweep
-//   xa[nall]   double4 (xi_x, xi_y, xi_z, alpha)   gathered in the force sweep
+//   pv[nall]   PosVel  (pos, vel), 64 B            gathered in the w sweep
+//   rec[nall]  ForceRec (pos, (w, rho), (xi, alpha)), one 128-B line per
+//              atom: the force sweep gathers one line per neighbour
 //   f, f_EPH, f_RNG: structure of arrays [3][nlocal] (streamed, not gathered)
 //   neighbour lists: transposed ELL, entry k of atom i at [k * stride + i],
 //     so lane i of a wavefront reads consecutive words (one thread per atom,
@@ -77,14 +78,24 @@ struct StepParams {
   const double* xi_in = nullptr;
 };
 
+// Per-atom record of the force sweep: position (with element), (w, rho) and
+// (xi, alpha) in one 128-B cache line; the fourth slot is unused.
+struct ForceRec {
+  double4 p, rw, xa, pad;
+};
+// Position and velocity side by side (64 B), gathered by the w sweep.
+struct PosVel {
+  double4 p, v;
+};
+
 // Work buffers of one post_force / end_of_step
 struct WorkView {
   const int* nbr = nullptr;   // full list, ELL
   const int* nnbr = nullptr;
   int* act = nullptr;         // active (r^2 < rc^2) entries, ELL, written by the density sweep
   int* nact = nullptr;
-  double4* rw = nullptr;
-  double4* xa = nullptr;
+  ForceRec* rec = nullptr;    // written by the density sweep (locals) and the ghost copy
+  PosVel* pv = nullptr;       // packed from pos, vel before the density sweep
   double* fE = nullptr;       // [3][fstride]
   double* fR = nullptr;       // [3][fstride]
   int* flags_out = nullptr;   // bit 0: rho > rho_cutoff seen
@@ -128,7 +139,8 @@ hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int nei
                              int* nn, int stride, hipStream_t s);
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
                             const WorkView& w, hipStream_t s);
-hipError_t launch_ghost_rw_xa(const AtomsView& a, const WorkView& w, hipStream_t s);
+hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s);
+hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s);
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
                              const StepParams& p, const WorkView& w, hipStream_t s);
 hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s);
@@ -159,10 +171,12 @@ hipError_t launch_peratom(const AtomsView& a, const TablesView& t, const WorkVie
                           double* out, hipStream_t s);
 // host-layout staging
 hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s);
-hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s);
+// in[i * stride4] for i < n (stride4 = 4 reads one field of ForceRec)
+hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s, int stride4 = 1);
 hipError_t launch_rows_into_soa(const double* rows, int n, double* soa, int stride, hipStream_t s);
 hipError_t launch_soa_into_rows(const double* soa, int n, int stride, double* rows, hipStream_t s);
-hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s);
+hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s,
+                               int stride4 = 1);
 // harness
 hipError_t launch_lj(const AtomsView& a, const int* nbr, const int* nn, int stride, double eps,
                      double sigma, double rc2, hipStream_t s);
@@ -187,8 +201,10 @@ hipError_t launch_fdm_substep_batch(const GridView* grids, const int* cell_rep, 
 hipError_t launch_copy_double(const double* src, double* dst, int n, hipStream_t s);
 hipError_t launch_ghost_xv_dev(const AtomsView& a, const int* owner, const double4* shift,
                                const int* ng, int cap, hipStream_t s);
-hipError_t launch_ghost_rw_xa_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
-                                  hipStream_t s);
+hipError_t launch_pack_pv_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                              hipStream_t s);
+hipError_t launch_ghost_rec_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                                hipStream_t s);
 hipError_t launch_step_advance(uint64_t* step, hipStream_t s);
 hipError_t launch_step_log(const int* flag, const uint64_t* step, const uint64_t* first, int* log,
                            int nlog, hipStream_t s);

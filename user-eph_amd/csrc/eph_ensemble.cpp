@@ -296,8 +296,9 @@ int eph_ensemble::enqueue_step(int nlog) {
   // post_force (fix_eph.cpp:841-907)
   StepParams p = sys->step_params();
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
+  EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
   EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s));
-  EPH_HIP(this, launch_ghost_rw_xa_dev(a, w, ng, ghost_cap, s));
+  EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   if (sys->model == 4)
     EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, p, w, chunks.p, nchunk, s));
   if ((rc = sys->dev_final_integrate())) return fail(rc, sys->err);

@@ -134,7 +134,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
     const double4 vi = a.vel[i];
     for (int k = 0; k < na; ++k) {
       const int j = w.act[k * stride + i];
-      const double4 pj = a.pos[j];
+      const double4 pj = w.pv[j].p;
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
@@ -142,7 +142,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
       const double pre = alpha * vr / (rho * r2);
       const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
       const double var1 = pre * ev1;
-      const double4 vj = a.vel[j];
+      const double4 vj = w.pv[j].v;
       const double ev2 = ex * vj.x + ey * vj.y + ez * vj.z;
       const double var2 = pre * ev2;
       const double dvar = var1 - var2;
@@ -161,8 +161,10 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
     gaussian3(p.seed, static_cast<uint32_t>(a.tag[i]), p.step, g3);
     x0 = g3[0]; x1 = g3[1]; x2 = g3[2];
   }
-  w.rw[i] = make_double4(wx, wy, wz, rho);
-  w.xa[i] = make_double4(x0, x1, x2, alpha);
+  ForceRec* r = w.rec + i;
+  r->p = pi;
+  r->rw = make_double4(wx, wy, wz, rho);
+  r->xa = make_double4(x0, x1, x2, alpha);
 }
 
 template <int XI, bool FRIC, bool MULTI>
@@ -210,13 +212,28 @@ __global__ __launch_bounds__(kBigBlock) void k_density_w_lds(AtomsView a, Tables
 }
 
 // forward communication of rho, w (fix_eph.cpp:743-744, 870-871) and xi
-// (:863-864) on one process: ghost <- owner
-__global__ __launch_bounds__(kBlock) void k_ghost_rw_xa(AtomsView a, WorkView w) {
-  const int g = blockIdx.x * kBlock + threadIdx.x;
-  if (g >= a.nghost) return;
+// (:863-864) on one process: ghost <- owner, next to the ghost's own position
+__device__ __forceinline__ void ghost_rec(const int g, const AtomsView& a, const WorkView& w) {
   const int o = w.ghost_owner[g];
-  w.rw[a.nlocal + g] = w.rw[o];
-  w.xa[a.nlocal + g] = w.xa[o];
+  ForceRec* r = w.rec + (a.nlocal + g);
+  r->p = a.pos[a.nlocal + g];
+  r->rw = w.rec[o].rw;
+  r->xa = w.rec[o].xa;
+}
+
+__global__ __launch_bounds__(kBlock) void k_ghost_rec(AtomsView a, WorkView w) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g < a.nghost) ghost_rec(g, a, w);
+}
+
+// positions and velocities of all atoms, packed for the w sweep
+__global__ __launch_bounds__(kBlock) void k_pack_pv(AtomsView a, WorkView w, int n) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  PosVel q;
+  q.p = a.pos[i];
+  q.v = a.vel[i];
+  w.pv[i] = q;
 }
 
 // ------------------------------------------- force_prl stages 2 and 3 fused
@@ -230,24 +247,25 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
                                                 const WorkView& w) {
   const int stride = a.stride;
   const bool grp = (a.mask[i] & a.groupbit) != 0;
-  const double4 rwi = w.rw[i];
+  const double4 rwi = w.rec[i].rw;
   const double rho_i = rwi.w;
   double fex = 0.0, fey = 0.0, fez = 0.0, frx = 0.0, fry = 0.0, frz = 0.0;
   if (grp && rho_i > 0.0) {
-    const double4 pi = a.pos[i];
-    const double4 xai = w.xa[i];
+    const double4 pi = w.rec[i].p;
+    const double4 xai = w.rec[i].xa;
     const double alpha_i = xai.w;
     const int ei = MULTI ? static_cast<int>(pi.w) : 0;
     const int na = w.nact[i];
     for (int k = 0; k < na; ++k) {
       const int j = w.act[k * stride + i];
-      const double4 rwj = w.rw[j];
+      const ForceRec* rj = w.rec + j;
+      const double4 rwj = rj->rw;
       const double rho_j = rwj.w;
       if (!(rho_j > 0.0)) continue;
-      const double4 pj = a.pos[j];
+      const double4 pj = rj->p;
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
-      const double4 xaj = w.xa[j];
+      const double4 xaj = rj->xa;
       const double alpha_j = xaj.w;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
       const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
@@ -787,7 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_peratom(AtomsView a, TablesView t, W
   if (i >= a.nlocal) return;
   double* o = out + 8 * static_cast<long>(i);
   if (a.mask[i] & a.groupbit) {
-    const double rho = w.rw[i].w;
+    const double rho = w.rec[i].rw.w;
     const int e = static_cast<int>(a.pos[i].w);
     o[0] = rho;
     o[1] = rho > t.rho_cut ? 0.0 : cubic_at(t.beta + e * t.n_beta, t.n_beta, t.inv_drho, rho);
@@ -966,13 +984,20 @@ __global__ __launch_bounds__(kBlock) void k_ghost_xv_dev(AtomsView a, const int*
   a.vel[a.nlocal + g] = a.vel[o];
 }
 
-__global__ __launch_bounds__(kBlock) void k_ghost_rw_xa_dev(AtomsView a, WorkView w,
-                                                            const int* __restrict__ ng) {
+__global__ __launch_bounds__(kBlock) void k_ghost_rec_dev(AtomsView a, WorkView w,
+                                                          const int* __restrict__ ng) {
   const int g = blockIdx.x * kBlock + threadIdx.x;
-  if (g >= *ng) return;
-  const int o = w.ghost_owner[g];
-  w.rw[a.nlocal + g] = w.rw[o];
-  w.xa[a.nlocal + g] = w.xa[o];
+  if (g < *ng) ghost_rec(g, a, w);
+}
+
+__global__ __launch_bounds__(kBlock) void k_pack_pv_dev(AtomsView a, WorkView w,
+                                                        const int* __restrict__ ng) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nlocal + *ng) return;
+  PosVel q;
+  q.p = a.pos[i];
+  q.v = a.vel[i];
+  w.pv[i] = q;
 }
 
 // ---------------------------------------------------- host layout staging
@@ -987,10 +1012,10 @@ __global__ __launch_bounds__(kBlock) void k_rows_into_d4(const double* __restric
 }
 
 __global__ __launch_bounds__(kBlock) void k_d4_into_rows(const double4* __restrict__ in, int n,
-                                                         double* __restrict__ rows) {
+                                                         int stride4, double* __restrict__ rows) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double4 o = in[i];
+  const double4 o = in[static_cast<long>(i) * stride4];
   rows[3 * i] = o.x; rows[3 * i + 1] = o.y; rows[3 * i + 2] = o.z;
 }
 
@@ -1009,10 +1034,11 @@ __global__ __launch_bounds__(kBlock) void k_soa_into_rows(const double* __restri
 }
 
 __global__ __launch_bounds__(kBlock) void k_d4_component(const double4* __restrict__ in, int n,
-                                                         int comp, double* __restrict__ out) {
+                                                         int stride4, int comp,
+                                                         double* __restrict__ out) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= n) return;
-  const double4 o = in[i];
+  const double4 o = in[static_cast<long>(i) * stride4];
   out[i] = comp == 0 ? o.x : comp == 1 ? o.y : comp == 2 ? o.z : o.w;
 }
 
@@ -1102,9 +1128,16 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   return hipGetLastError();
 }
 
-hipError_t launch_ghost_rw_xa(const AtomsView& a, const WorkView& w, hipStream_t s) {
+hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s) {
+  const int n = a.nlocal + a.nghost;
+  if (n == 0) return hipSuccess;
+  k_pack_pv<<<nblk(n), kBlock, 0, s>>>(a, w, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s) {
   if (a.nghost == 0) return hipSuccess;
-  k_ghost_rw_xa<<<nblk(a.nghost), kBlock, 0, s>>>(a, w);
+  k_ghost_rec<<<nblk(a.nghost), kBlock, 0, s>>>(a, w);
   return hipGetLastError();
 }
 
@@ -1284,9 +1317,9 @@ hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStrea
   k_rows_into_d4<<<nblk(n), kBlock, 0, s>>>(rows, n, out);
   return hipGetLastError();
 }
-hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s) {
+hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s, int stride4) {
   if (n == 0) return hipSuccess;
-  k_d4_into_rows<<<nblk(n), kBlock, 0, s>>>(in, n, rows);
+  k_d4_into_rows<<<nblk(n), kBlock, 0, s>>>(in, n, stride4, rows);
   return hipGetLastError();
 }
 hipError_t launch_rows_into_soa(const double* rows, int n, double* soa, int stride, hipStream_t s) {
@@ -1299,9 +1332,10 @@ hipError_t launch_soa_into_rows(const double* soa, int n, int stride, double* ro
   k_soa_into_rows<<<nblk(n), kBlock, 0, s>>>(soa, n, stride, rows);
   return hipGetLastError();
 }
-hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s) {
+hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, hipStream_t s,
+                               int stride4) {
   if (n == 0) return hipSuccess;
-  k_d4_component<<<nblk(n), kBlock, 0, s>>>(in, n, comp, out);
+  k_d4_component<<<nblk(n), kBlock, 0, s>>>(in, n, stride4, comp, out);
   return hipGetLastError();
 }
 
@@ -1381,10 +1415,18 @@ hipError_t launch_ghost_xv_dev(const AtomsView& a, const int* owner, const doubl
   return hipGetLastError();
 }
 
-hipError_t launch_ghost_rw_xa_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
-                                  hipStream_t s) {
+hipError_t launch_pack_pv_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                              hipStream_t s) {
+  const long n = static_cast<long>(a.nlocal) + cap;
+  if (n == 0) return hipSuccess;
+  k_pack_pv_dev<<<nblk(n), kBlock, 0, s>>>(a, w, ng);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_rec_dev(const AtomsView& a, const WorkView& w, const int* ng, int cap,
+                                hipStream_t s) {
   if (cap == 0) return hipSuccess;
-  k_ghost_rw_xa_dev<<<nblk(cap), kBlock, 0, s>>>(a, w, ng);
+  k_ghost_rec_dev<<<nblk(cap), kBlock, 0, s>>>(a, w, ng);
   return hipGetLastError();
 }
 
