@@ -1,0 +1,120 @@
+"""GPU: trajectory gradients (include/eph_adjoint.h) against finite differences.
+
+Loss L = Ee(after n steps) + sum_c (T_e,c - T_target,c)^2 of the device run
+(harness + fix, Philox noise fixed by the seed).  The adjoint gives dL/dbeta
+knots and dL/dv0; directional derivatives along random directions are compared
+with central differences of the same device run, and with central differences
+of the oracle (the CPU FixEPH restatement driven through the host harness).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import eph_amd as ea
+from eph_amd import inputs
+from eph_amd.adjoint_worker import build_trajectory
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+SMALL = dict(cells=4, dt=5e-4, nsteps=12, every=5, T=600.0, pka_eV=100.0, target_seed=1,
+             n_beta=64, md_seed=7, fix_seed=4711)
+H = 1e-6
+
+
+def _central(f, x, d, h):
+    return (f(x + h * d) - f(x - h * d)) / (2.0 * h)
+
+
+@pytest.fixture(scope="module")
+def small(tmp_path_factory):
+    spec = dict(SMALL, tmpdir=str(tmp_path_factory.mktemp("adj")))
+    md, fix, adj, meta = build_trajectory(spec)
+    loss0 = adj.forward()
+    gb, gv = adj.backward()
+    yield dict(md=md, fix=fix, adj=adj, meta=meta, loss=loss0, gb=gb, gv=gv, spec=spec)
+    adj.close()
+
+
+def test_forward_is_the_device_run(small):
+    """the adjoint's forward pass is the ordinary device-resident run"""
+    spec, adj = small["spec"], small["adj"]
+    md = ea.MD(spec["cells"], ntypes=1, seed=spec["md_seed"], T=spec["T"], pka_eV=spec["pka_eV"])
+    inf = md.info()
+    fix = ea.FixEPH(small["meta"]["args"], 1, inf["boxlo"], inf["boxhi"], spec["dt"], inputs.BOLTZ,
+                    inputs.FTM2V, masses=[inputs.MASS_CU])
+    fix.init()
+    md.attach(fix)
+    md.run(fix, spec["nsteps"], first_step=1)
+    T = fix.grid().get("T")
+    tgt = small["meta"]["target"]
+    loss, ee, sq = small["loss"]
+    assert ee == fix.compute_vector(0)
+    assert sq == sum(((T - tgt) ** 2).tolist())  # sequential, as the library sums
+    assert adj.forward() == small["loss"]  # repeatable
+
+
+def test_beta_gradient_matches_central_differences(small):
+    adj, b0, gb = small["adj"], small["adj"].beta0(), small["gb"]
+    rng = np.random.default_rng(11)
+    for _ in range(2):
+        d = rng.standard_normal(b0.shape) * b0  # relative perturbation of each knot
+        fd = _central(lambda b: adj.forward(beta=b)[0], b0, d, H)
+        ad = float((gb * d).sum())
+        assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(small["loss"][0]), (fd, ad)
+
+
+def test_velocity_gradient_matches_central_differences(small):
+    adj, gv = small["adj"], small["gv"]
+    v0 = adj.v0()
+    rng = np.random.default_rng(12)
+    scale = np.abs(v0).mean()
+    for _ in range(2):
+        d = rng.standard_normal(v0.shape) * scale
+        fd = _central(lambda v: adj.forward(v0=v)[0], v0, d, H)
+        ad = float((gv * d).sum())
+        assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(small["loss"][0]), (fd, ad)
+    adj.forward()
+
+
+def test_beta_gradient_matches_cpu_reference(small, oracle, tmp_path):
+    """central differences of the oracle trajectory (host harness, CPU FixEPH)"""
+    spec, meta, gb = small["spec"], small["meta"], small["gb"]
+    b0 = small["adj"].beta0()
+    d = np.random.default_rng(13).standard_normal(b0.shape) * b0
+    header = open(meta["beta_file"]).read().splitlines()
+
+    def oracle_loss(beta, tag):
+        # the same .beta with other beta knots (rho(r) lines unchanged)
+        n_rho = int(header[4].split()[0])
+        lines = header[:6 + n_rho] + [repr(float(v)) for v in beta[0]]
+        path = str(tmp_path / f"b{tag}.beta")
+        open(path, "w").write("\n".join(lines) + "\n")
+        args = list(meta["args"])
+        args[16] = path
+        md = ea.MD(spec["cells"], ntypes=1, seed=spec["md_seed"], T=spec["T"], pka_eV=spec["pka_eV"])
+        ora = oracle.Fix(args, 1, meta["boxlo"], meta["boxhi"], spec["dt"], inputs.BOLTZ,
+                         inputs.FTM2V, masses=[inputs.MASS_CU])
+        s = md.snapshot()
+        md.lj(s)
+        for n in range(1, spec["nsteps"] + 1):
+            ora.initial_integrate(s)
+            if md.needs_rebuild(s):
+                md.rebuild(s)
+                v = s.v[:s.nlocal].copy()
+                s = md.snapshot()
+                s.v[:s.nlocal] = v
+            md.update_ghosts(s)
+            md.lj(s)
+            ora.post_force(s, step=n)
+            ora.final_integrate(s)
+            ora.end_of_step(s, n)
+        T = ora.fdm().get("T")
+        return ora.compute_vector(0) + sum(((T - meta["target"]) ** 2).tolist())
+
+    assert oracle_loss(b0, "0") == small["loss"][0]  # the oracle trajectory is the device one
+    fd = (oracle_loss(b0 + H * d, "p") - oracle_loss(b0 - H * d, "m")) / (2 * H)
+    ad = float((gb * d).sum())
+    assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(small["loss"][0]), (fd, ad)

@@ -200,3 +200,22 @@ def test_neighbour_rebuild_after_large_moves():
     md.rebuild(s)
     s2 = md.snapshot()
     assert np.all(s2.x[:s2.nlocal] >= s2.boxlo) and np.all(s2.x[:s2.nlocal] < s2.boxhi)
+
+
+def test_spline_jacobian_matches_finite_differences():
+    """d(table)/d(knots) of the Akima build, used by the beta-knot gradient"""
+    rng = np.random.default_rng(21)
+    for n, dx in ((8, 0.3), (64, 20.0 / 63)):
+        x = np.arange(n) * dx
+        y = np.sqrt(0.02 * x / (1 + 0.5 * x)) + 0.01 * rng.standard_normal(n)
+        J = ea.spline_jacobian(dx, y)
+        for m in range(n):
+            h = 1e-6 * max(abs(y[m]), 1e-3)
+            yp, ym = y.copy(), y.copy()
+            yp[m] += h
+            ym[m] -= h
+            fd = (ea.spline_build(dx, yp) - ea.spline_build(dx, ym)) / (2 * h)
+            scale = np.abs(J[:, :, m]).max()
+            assert np.abs(fd - J[:, :, m]).max() <= 1e-6 * scale, m
+    # the top knot's constant piece: only y[n-1], with weight 1
+    assert J[-1, 0, -1] == 1.0 and np.count_nonzero(J[-1]) == 1

@@ -417,6 +417,27 @@ int eph_context::dev_post_force() {
 }
 
 int eph_context::dev_end_of_step(long long ntimestep) {
+  int rc = dev_deposit();
+  if (rc != EPH_OK) return rc;
+  hipEvent_t e;
+  if (flags & EPH_FDM) {
+    timed_begin("fdm_solve", &e);
+    rc = grid->solve(nullptr);
+    if (rc != EPH_OK) return fail(rc, grid->err);
+    timed_end("fdm_solve", e);
+  }
+  if (T_freq > 0 && (ntimestep % T_freq) == 0) {
+    std::vector<double> T;
+    rc = grid->download(0, T);
+    if (rc != EPH_OK) return fail(rc, grid->err);
+    if (!save_grid_temperature(grid->h, T, T_out, (int)(ntimestep / T_freq)))
+      return fail(EPH_ERR_FILE, "cannot write " + T_out);
+  }
+  return EPH_OK;
+}
+
+// the deposit half of end_of_step (fix_eph.cpp:364-404): dT_e and the tally
+int eph_context::dev_deposit() {
   if (!have_atoms) return fail(EPH_ERR_STATE, "no atoms");
   const bool fric = flags & EPH_FRICTION, rnd = flags & EPH_RANDOM;
   const int nb = nblocks_of(nlocal);
@@ -436,19 +457,6 @@ int eph_context::dev_end_of_step(long long ntimestep) {
     timed_end("deposit", e);
   } else {
     EPH_HIP(this, launch_fill_double(scal.p, 1, 0.0, stream));
-  }
-  if (flags & EPH_FDM) {
-    timed_begin("fdm_solve", &e);
-    int rc = grid->solve(nullptr);
-    if (rc != EPH_OK) return fail(rc, grid->err);
-    timed_end("fdm_solve", e);
-  }
-  if (T_freq > 0 && (ntimestep % T_freq) == 0) {
-    std::vector<double> T;
-    int rc = grid->download(0, T);
-    if (rc != EPH_OK) return fail(rc, grid->err);
-    if (!save_grid_temperature(grid->h, T, T_out, (int)(ntimestep / T_freq)))
-      return fail(EPH_ERR_FILE, "cannot write " + T_out);
   }
   return EPH_OK;
 }

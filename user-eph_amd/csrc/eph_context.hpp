@@ -21,6 +21,15 @@ struct DevArray {
   DevArray() = default;
   DevArray(const DevArray&) = delete;
   DevArray& operator=(const DevArray&) = delete;
+  DevArray(DevArray&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+  DevArray& operator=(DevArray&& o) noexcept {
+    if (this != &o) {
+      release();
+      p = o.p; n = o.n;
+      o.p = nullptr; o.n = 0;
+    }
+    return *this;
+  }
   ~DevArray() { release(); }
   void release() {
     if (p) (void)hipFree(p);
@@ -150,6 +159,7 @@ struct eph_context {
   int dev_post_force();
   int dev_final_integrate();
   int dev_end_of_step(long long ntimestep);
+  int dev_deposit();
   int check_flags();
   // timing helpers
   hipEvent_t take_event();

@@ -1,0 +1,60 @@
+// eph_devfn.hpp -- small device helpers shared by the kernel files.
+#pragma once
+
+#include "eph_device.hpp"
+
+namespace eph {
+
+// Blocks are dealt round-robin to the 8 XCDs (block b -> XCD b % 8).  Remap so
+// that each XCD owns one contiguous range of atoms: neighbour gathers of a
+// range then stay in that XCD's L2.
+static __device__ __forceinline__ int xcd_block() {
+  constexpr int kXcd = 8;
+  const int nb = gridDim.x, b = blockIdx.x;
+  const int per = nb / kXcd, rem = nb % kXcd;
+  const int x = b % kXcd, local = b / kXcd;
+  return x * per + (x < rem ? x : rem) + local;
+}
+
+static __device__ __forceinline__ double cubic_at(const double4* __restrict__ tab, int n, double inv,
+                                           double x) {
+  int i = static_cast<int>(x * inv);
+  i = i < 0 ? 0 : (i < n - 1 ? i : n - 1);
+  const double4 c = tab[i];
+  return c.x + x * (c.y + x * (c.z + x * c.w));
+}
+
+// EPH_FDM::get_index (eph_fdm.h:494-509): floor, then periodic fold
+static __device__ __forceinline__ int cell_of(const GridView& g, double x, double y, double z) {
+  int lx = static_cast<int>(floor((x - g.x0) / g.dx));
+  int px = static_cast<int>(floor(static_cast<double>(lx) / g.nx));
+  lx -= px * g.nx;
+  int ly = static_cast<int>(floor((y - g.y0) / g.dy));
+  int py = static_cast<int>(floor(static_cast<double>(ly) / g.ny));
+  ly -= py * g.ny;
+  int lz = static_cast<int>(floor((z - g.z0) / g.dz));
+  int pz = static_cast<int>(floor(static_cast<double>(lz) / g.nz));
+  lz -= pz * g.nz;
+  const int c = lx + ly * g.nx + lz * g.nx * g.ny;
+  // finite coordinates always land in range; non-finite ones (a run that has
+  // blown up) must not index outside the grid
+  return (c >= 0 && c < g.nx * g.ny * g.nz) ? c : 0;
+}
+
+template <typename T>
+static __device__ __forceinline__ T block_sum(T v, T* sh) {
+  // fixed-order tree: deterministic for a given block size
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] = sh[t] + sh[t + s];
+    __syncthreads();
+  }
+  T r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+
+}  // namespace eph

@@ -107,6 +107,8 @@ bool load_beta(const std::string& path, BetaModel& m, std::string& err) {
   m.rho_cutoff = drho * (nb - 1);
   const double dr_sq = m.r_cutoff_sq / (nr - 1);
   m.rho.resize(ne); m.rho_r_sq.resize(ne); m.beta.resize(ne); m.alpha.resize(ne);
+  m.drho = drho;
+  m.beta_y.assign(ne, std::vector<double>());
   m.numbers.assign(ne, 0);
   std::vector<double> knots_r(nr), knots_b(nb);
   for (size_t e = 0; e < ne; ++e) {
@@ -125,6 +127,7 @@ bool load_beta(const std::string& path, BetaModel& m, std::string& err) {
     for (size_t j = 0; j < nr; ++j) knots_r2[j] = m.rho[e](std::sqrt(j * dr_sq));
     if (!build_spline(dr_sq, knots_r2, m.rho_r_sq[e], err)) return false;
     if (!build_spline(drho, knots_b, m.beta[e], err)) return false;
+    m.beta_y[e] = knots_b;
     // alpha knots are sqrt(beta) knots (eph_beta.h:120-124)
     std::vector<double> knots_a(nb);
     for (size_t j = 0; j < nb; ++j) knots_a[j] = std::sqrt(knots_b[j]);

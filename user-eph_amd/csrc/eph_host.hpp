@@ -44,6 +44,8 @@ struct BetaModel {
   double r_cutoff = 0, r_cutoff_sq = 0, rho_cutoff = 0;
   int n_rho = 0, n_beta = 0;
   std::vector<Spline> rho, rho_r_sq, beta, alpha;
+  double drho = 0;                           // beta knot spacing
+  std::vector<std::vector<double>> beta_y;   // beta knot values per element
 };
 bool load_beta(const std::string& path, BetaModel& m, std::string& err);
 

@@ -8,6 +8,7 @@ is missing or no HIP device is present, the calls raise.
     Grid    -- the electronic T_e grid (EPH_FDM, eph_fdm.h), also standalone
     MD      -- the minimal MD harness standing in for LAMMPS (include/eph_md.h)
     Ensemble -- many replicas advanced together (include/eph_ensemble.h)
+    Adjoint -- trajectory gradients w.r.t. beta knots and v0 (include/eph_adjoint.h)
 """
 import ctypes as C
 import os
@@ -123,6 +124,15 @@ _SIG = {
     "eph_ensemble_size": (C.c_int, [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
     "eph_ensemble_info": (C.c_int, [vp, C.c_int, ip, ip, ip]),
     "eph_ensemble_get": (C.c_int, [vp, C.c_int, C.c_int, dp]),
+    "eph_adjoint_create": (C.c_int, [C.POINTER(vp), vp, vp, C.c_int, C.c_int, dp]),
+    "eph_adjoint_destroy": (None, [vp]),
+    "eph_adjoint_last_error": (C.c_char_p, [vp]),
+    "eph_adjoint_info": (C.c_int, [vp, ip, ip, ip, ip]),
+    "eph_adjoint_get_beta": (C.c_int, [vp, dp]),
+    "eph_adjoint_get_v0": (C.c_int, [vp, dp]),
+    "eph_adjoint_forward": (C.c_int, [vp, dp, dp, dp, dp]),
+    "eph_adjoint_backward": (C.c_int, [vp, dp, dp]),
+    "eph_spline_jacobian": (C.c_int, [C.c_double, dp, C.c_int, dp]),
 }
 
 _lib = None
@@ -601,6 +611,73 @@ class Ensemble:
         out = np.zeros((inf["nlocal"], 3)) if k < 3 else np.zeros(inf["cells"] if k == 3 else 2)
         self._check(lib().eph_ensemble_get(self.h, r, k, _d(out)))
         return out
+
+
+def spline_jacobian(dx, y):
+    """d(table coefficients)/d(knot values) of spline_build: shape (n, 4, n)."""
+    y = f64(y)
+    n = len(y)
+    out = np.zeros((n, 4, n))
+    rc = lib().eph_spline_jacobian(dx, _d(y), n, _d(out))
+    if rc != EPH_OK:
+        raise EphError(rc, "spline Jacobian")
+    return out
+
+
+class Adjoint:
+    """Loss L = Ee(after nsteps) + sum_c (T_e,c - T_target,c)^2 of the device run of
+    `md` with `fix` (attached), and its gradient w.r.t. the beta knots and the initial
+    velocities (include/eph_adjoint.h).  Every forward starts from the state at creation."""
+
+    def __init__(self, md, fix, nsteps, every=20, T_target=None):
+        h = vp()
+        tt = None if T_target is None else f64(T_target)
+        rc = lib().eph_adjoint_create(C.byref(h), md.h, fix.h, nsteps, every,
+                                      None if tt is None else _d(tt))
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_adjoint_last_error(None).decode())
+        self.h = h
+        self._keep = (md, fix)
+        ne, nb, nl, nc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        lib().eph_adjoint_info(h, C.byref(ne), C.byref(nb), C.byref(nl), C.byref(nc))
+        self.n_elem, self.n_beta, self.nlocal, self.ncells = ne.value, nb.value, nl.value, nc.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().eph_adjoint_destroy(self.h)
+            self.h = None
+
+    __del__ = close
+
+    def _check(self, rc):
+        if rc != EPH_OK:
+            raise EphError(rc, lib().eph_adjoint_last_error(self.h).decode())
+
+    def beta0(self):
+        out = np.zeros((self.n_elem, self.n_beta))
+        self._check(lib().eph_adjoint_get_beta(self.h, _d(out)))
+        return out
+
+    def v0(self):
+        out = np.zeros((self.nlocal, 3))
+        self._check(lib().eph_adjoint_get_v0(self.h, _d(out)))
+        return out
+
+    def forward(self, beta=None, v0=None):
+        """(loss, Ee, grid term) of the trajectory with these beta knots / start velocities"""
+        b = None if beta is None else f64(beta)
+        v = None if v0 is None else f64(v0)
+        loss, parts = C.c_double(), np.zeros(2)
+        self._check(lib().eph_adjoint_forward(self.h, None if b is None else _d(b),
+                                              None if v is None else _d(v), C.byref(loss), _d(parts)))
+        return loss.value, parts[0], parts[1]
+
+    def backward(self):
+        """(dL/dbeta [n_elem, n_beta], dL/dv0 [nlocal, 3]) of the last forward"""
+        gb = np.zeros((self.n_elem, self.n_beta))
+        gv = np.zeros((self.nlocal, 3))
+        self._check(lib().eph_adjoint_backward(self.h, _d(gb), _d(gv)))
+        return gb, gv
 
 
 def exported_symbols(header_paths):

@@ -54,6 +54,20 @@ def synthetic_beta(path, amplitudes=(2.0,), prefactors=(0.02,), names=None, n_kn
     return names
 
 
+def adjoint_beta(path, n_beta=64, amplitude=2.0, prefactor=0.02, n_rho=1000, r_cut=4.5,
+                 r_decay=0.9, rho_max=20.0):
+    """One element: rho(r) as synthetic_beta (1000 knots), beta(rho) = B rho/(1 + 0.5 rho)
+    on n_beta knots up to rho_max (the parameterisation the adjoint differentiates)."""
+    dr = r_cut / (n_rho - 1)
+    drho = rho_max / (n_beta - 1)
+    r = np.arange(n_rho) * dr
+    rho_grid = np.arange(n_beta) * drho
+    write_beta(path, ["Cu"], [29], n_rho, dr, n_beta, drho, r_cut,
+               [amplitude * np.exp(-r / r_decay)], [prefactor * rho_grid / (1.0 + 0.5 * rho_grid)],
+               comment=f"synthetic, {n_beta} beta knots")
+    return ["Cu"]
+
+
 def write_params(path, dT, C_e, kappa):
     with open(path, "w") as f:
         f.write("#\n#\n#\n")
