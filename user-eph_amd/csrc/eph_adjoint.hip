@@ -167,7 +167,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_lj(AdjStepView s, AtomsView a, d
     const double r2i = 1.0 / r2;
     const double r4i = r2i * r2i, r6i = r4i * r2i, r8i = r4i * r4i;
     const double F = r6i * (lj1 * r6i - lj2) * r2i;
-    const double dF = -7.0 * lj1 * r8i * r8i + 4.0 * lj2 * r4i * r2i;
+    const double dF = -7.0 * lj1 * r8i * r8i + 4.0 * lj2 * r8i * r2i;
     const double ud = u.x * dx + u.y * dy + u.z * dz;
     const double gx = F * u.x + 2.0 * dF * ud * dx;
     const double gy = F * u.y + 2.0 * dF * ud * dy;
