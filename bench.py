@@ -175,6 +175,53 @@ def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
     return out
 
 
+def adjoint_bench(tmp, nsteps, every):
+    """Trajectory gradient: fcc 20^3 (32,000 atoms), 600 K, 2 keV PKA, 64 beta knots,
+    dt 0.5 fs, loss = Ee + sum (T_e - T_target)^2; forward and backward timed, and
+    the adjoint checked against central differences along one random direction in
+    beta and one in v0 (h = 1e-6 relative)."""
+    import numpy as np
+    from eph_amd.adjoint_worker import build_trajectory
+    out = None
+    for dt in (5e-4, 2.5e-4):
+        spec = dict(cells=20, dt=dt, nsteps=nsteps, every=every, T=600.0, pka_eV=2000.0,
+                    target_seed=1, n_beta=64, md_seed=2020, fix_seed=5150, tmpdir=tmp)
+        md, fix, adj, _ = build_trajectory(spec)
+        try:
+            t0 = time.perf_counter()
+            loss = adj.forward()
+            t_fwd = time.perf_counter() - t0
+        except Exception as e:  # a trajectory that blows up at this dt
+            log(f"adjoint trajectory at dt {dt}: {e}")
+            adj.close()
+            continue
+        t0 = time.perf_counter()
+        gb, gv = adj.backward()
+        t_bwd = time.perf_counter() - t0
+        rng = np.random.default_rng(17)
+        b0, v0 = adj.beta0(), adj.v0()
+        h = 1e-6
+        db = rng.standard_normal(b0.shape) * b0
+        fd_b = (adj.forward(beta=b0 + h * db)[0] - adj.forward(beta=b0 - h * db)[0]) / (2 * h)
+        dv = rng.standard_normal(v0.shape) * np.abs(v0).mean()
+        fd_v = (adj.forward(v0=v0 + h * dv)[0] - adj.forward(v0=v0 - h * dv)[0]) / (2 * h)
+        ad_b, ad_v = float((gb * db).sum()), float((gv * dv).sum())
+        out = {
+            "workload": "fcc20_pka2keV_adjoint", "atoms": adj.nlocal, "Te_cells": adj.ncells,
+            "steps": nsteps, "checkpoint_every": every, "dt_ps": dt, "beta_knots": adj.n_beta,
+            "loss": loss[0], "Ee": loss[1], "grid_term": loss[2],
+            "forward_ms": t_fwd * 1e3, "backward_ms": t_bwd * 1e3,
+            "backward_over_forward": t_bwd / t_fwd,
+            "fd_check_beta": {"adjoint": ad_b, "central_difference": fd_b,
+                              "rel_err": abs(ad_b - fd_b) / max(abs(fd_b), 1e-300)},
+            "fd_check_v0": {"adjoint": ad_v, "central_difference": fd_v,
+                            "rel_err": abs(ad_v - fd_v) / max(abs(fd_v), 1e-300)},
+        }
+        adj.close()
+        break
+    return out or {"error": "trajectory not finite at dt 0.5 fs or 0.25 fs"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -191,6 +238,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=ENS_STEPS)
     ap.add_argument("--no-sequential", action="store_true",
                     help="skip the one-replica-at-a-time comparison of the ensemble line")
+    ap.add_argument("--adjoint-steps", type=int, default=200,
+                    help="steps of the trajectory-gradient line (0 = skip it)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -331,6 +380,8 @@ def main():
                                              args.warmup, not args.no_sequential)
         except ea.EphError as e:
             out["ensemble"] = {"error": str(e)}
+    if args.adjoint_steps > 0:
+        out["adjoint"] = adjoint_bench(tmp, args.adjoint_steps, 20)
     print(json.dumps(out), flush=True)
 
 
