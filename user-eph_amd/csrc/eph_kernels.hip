@@ -51,10 +51,17 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   double rho = 0.0;
   int na = 0;
   if (grp) {
+    // each neighbour's position is loaded one iteration ahead of its use
     const int nn = w.nnbr[i];
+    int jn = nn > 0 ? w.nbr[i] : 0;
+    double4 pn = nn > 0 ? a.pos[jn] : pi;
     for (int k = 0; k < nn; ++k) {
-      const int j = w.nbr[k * stride + i];
-      const double4 pj = a.pos[j];
+      const int j = jn;
+      const double4 pj = pn;
+      if (k + 1 < nn) {
+        jn = w.nbr[(k + 1) * stride + i];
+        pn = a.pos[jn];
+      }
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
       if (r2 < t.rc2) {
@@ -78,9 +85,19 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   double wx = 0.0, wy = 0.0, wz = 0.0;
   if (FRIC && grp && rho > 0.0) {
     const double4 vi = a.vel[i];
+    int jn = na > 0 ? w.act[i] : 0;
+    double4 pn = pi, vn = vi;
+    if (na > 0) {
+      pn = w.pv[jn].p;
+      vn = w.pv[jn].v;
+    }
     for (int k = 0; k < na; ++k) {
-      const int j = w.act[k * stride + i];
-      const double4 pj = w.pv[j].p;
+      const double4 pj = pn, vj = vn;
+      if (k + 1 < na) {
+        jn = w.act[(k + 1) * stride + i];
+        pn = w.pv[jn].p;
+        vn = w.pv[jn].v;
+      }
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
@@ -88,7 +105,6 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
       const double pre = alpha * vr / (rho * r2);
       const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
       const double var1 = pre * ev1;
-      const double4 vj = w.pv[j].v;
       const double ev2 = ex * vj.x + ey * vj.y + ez * vj.z;
       const double var2 = pre * ev2;
       const double dvar = var1 - var2;
@@ -202,16 +218,22 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
     const double alpha_i = xai.w;
     const int ei = MULTI ? static_cast<int>(pi.w) : 0;
     const int na = w.nact[i];
+    // each neighbour's record is loaded one iteration ahead of its use
+    int jn = na > 0 ? w.act[i] : 0;
+    double4 nrw = rwi, np = pi, nxa = xai;
+    if (na > 0) {
+      nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+    }
     for (int k = 0; k < na; ++k) {
-      const int j = w.act[k * stride + i];
-      const ForceRec* rj = w.rec + j;
-      const double4 rwj = rj->rw;
+      const double4 rwj = nrw, pj = np, xaj = nxa;
+      if (k + 1 < na) {
+        jn = w.act[(k + 1) * stride + i];
+        nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+      }
       const double rho_j = rwj.w;
       if (!(rho_j > 0.0)) continue;
-      const double4 pj = rj->p;
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
-      const double4 xaj = rj->xa;
       const double alpha_j = xaj.w;
       const int ej = MULTI ? static_cast<int>(pj.w) : 0;
       const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
@@ -777,9 +799,14 @@ __global__ __launch_bounds__(kBlock) void k_lj(AtomsView a, const int* __restric
   const double4 pi = a.pos[i];
   double fx = 0.0, fy = 0.0, fz = 0.0;
   const int n = nn[i];
+  int jn = n > 0 ? nbr[i] : 0;
+  double4 pn = n > 0 ? a.pos[jn] : pi;
   for (int k = 0; k < n; ++k) {
-    const int j = nbr[k * stride + i];
-    const double4 pj = a.pos[j];
+    const double4 pj = pn;
+    if (k + 1 < n) {
+      jn = nbr[(k + 1) * stride + i];
+      pn = a.pos[jn];
+    }
     const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
     const double r2 = dx * dx + dy * dy + dz * dz;
     if (r2 < rc2) {
