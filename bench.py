@@ -127,8 +127,10 @@ def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
     sub = ens.info(0)["substeps"]
     log(f"ensemble: {nrep} replicas, {size['atoms']} atoms, {size['cells']} cells, "
         f"{sub} sub-steps ({time.time() - t0:.1f} s)")
-    # launch mode: recorded graph or direct launches, whichever is faster on
-    # a short trial (both give the same bits)
+    # kernel forms of the two neighbour sweeps, then the launch mode (recorded
+    # graph or direct launches), each the faster on a short trial; every
+    # choice gives the same bits
+    tuned = ens.autotune()
     trial = {}
     for g in (True, False):
         trial[g] = ens.run(max(warmup, 2), graph=g, log=False)["ms"]
@@ -141,6 +143,7 @@ def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
         "replicas": nrep, "atoms": size["atoms"], "Te_cells": size["cells"],
         "steps": nsteps, "dt_ps": ENS_DT, "substeps": sub, "pka_eV": 1000.0,
         "launch": "graph" if use_graph else "direct",
+        "autotune": tuned,
         "trial_ms_per_step": {"graph": trial[True] / max(warmup, 2),
                               "direct": trial[False] / max(warmup, 2)},
         "ms_per_step": res["ms"] / nsteps,

@@ -54,6 +54,17 @@ int eph_ensemble_setup(eph_ensemble* e, long long first_step);
  * rebuilt[k] (nsteps entries, may be NULL) is 1 when step first + k rebuilt. */
 int eph_ensemble_run(eph_ensemble* e, int nsteps, int use_graph, double* ms, int* rebuilt);
 
+/* Time the two forms of the density and force sweeps (0: one chunk per
+ * 256-thread block, tables in global memory; 1: persistent 1024-thread blocks,
+ * tables in LDS) on the current system and keep the faster for later steps.
+ * variants[2] = chosen forms (density, force); ms[4] = average times of
+ * density form 0, 1, force form 0, 1 (may be NULL).  Both forms give the same
+ * bits; the state of the replicas is unchanged. */
+int eph_ensemble_autotune(eph_ensemble* e, int* variants, double* ms);
+
+/* pin the forms instead (0 or 1 each) */
+int eph_ensemble_set_variants(eph_ensemble* e, int density, int force);
+
 /* replicas, total local atoms, total T_e cells */
 int eph_ensemble_size(const eph_ensemble* e, int* nrep, long long* natoms, long long* ncells);
 /* replica r: local atoms, T_e cells, FDM sub-steps per MD step */

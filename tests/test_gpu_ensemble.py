@@ -123,3 +123,25 @@ def test_ensemble_fdm_substeps_differ_per_replica(tmp_path):
     for r, (kw, args) in enumerate(reps):
         md, fix = _separate(kw, args, species, dt, res["rebuilt"])
         assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), r
+
+
+def test_ensemble_sweep_forms_give_the_same_bits(tmp_path):
+    """the chunk-per-block and persistent LDS-table forms of the sweeps, and the
+    autotuner's choice, all give the same trajectory"""
+    species, dt = 2, 5e-4
+    reps = _replicas(str(tmp_path), SPECS, species, dt)
+    runs = []
+    for forms in (("chunk", "chunk"), ("lds", "lds"), ("chunk", "lds"), None):
+        ens = _ensemble(reps, species, dt)
+        if forms is None:
+            tuned = ens.autotune()
+            assert tuned["density"] in ("chunk", "lds") and tuned["force"] in ("chunk", "lds")
+        else:
+            ens.set_variants(*forms)
+        ens.run(15, graph=True)
+        runs.append([(ens.get(r, "x"), ens.get(r, "v"), ens.get(r, "T"), ens.get(r, "E"))
+                     for r in range(len(reps))])
+    for other in runs[1:]:
+        for r in range(len(reps)):
+            for q in range(4):
+                assert np.array_equal(runs[0][r][q], other[r][q]), (r, q)

@@ -184,12 +184,17 @@ hipError_t launch_max_disp2(const AtomsView& a, const double4* ref, double* part
                             hipStream_t s);
 hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream_t s);
 // batched replicas (one launch per phase for all replicas)
+// variant 0: one chunk per 256-thread block, tables from global memory;
+// variant 1: persistent 1024-thread blocks with the tables in LDS (falls back
+// to 0 when the tables do not fit).  Both give the same bits.
+bool batch_lds_possible(const TablesView& t);
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w, const Chunk* chunks, int nchunks,
-                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s);
+                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s,
+                                  int variant = 0);
 hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, const GridView* grids,
                                    const StepParams& p, const WorkView& w, const Chunk* chunks,
-                                   int nchunks, hipStream_t s);
+                                   int nchunks, hipStream_t s, int variant = 0);
 hipError_t launch_deposit_atoms_batch(const AtomsView& a, const GridView* grids, const int* cell_off,
                                       const StepParams& p, const WorkView& w, const DepositView& d,
                                       const Chunk* chunks, int nchunks, hipStream_t s);

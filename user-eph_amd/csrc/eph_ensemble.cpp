@@ -65,6 +65,7 @@ struct eph_ensemble {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_nlog = -1;
+  int var_density = 0, var_force = 0;  // kernel forms chosen by autotune()
 
   int fail(int code, const std::string& m) { err = m; return code; }
   int fail_hip(hipError_t e, const char* what) {
@@ -83,6 +84,7 @@ struct eph_ensemble {
   int setup(long long first);
   int enqueue_step(int nlog);
   int record(int nlog);
+  int autotune(double* ms);
 };
 
 int eph_ensemble::setup(long long first) {
@@ -297,10 +299,12 @@ int eph_ensemble::enqueue_step(int nlog) {
   StepParams p = sys->step_params();
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
   EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
-  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s));
+  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s,
+                                       var_density));
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   if (sys->model == 4)
-    EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, p, w, chunks.p, nchunk, s));
+    EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, p, w, chunks.p, nchunk, s,
+                                          var_force));
   if ((rc = sys->dev_final_integrate())) return fail(rc, sys->err);
   // end_of_step (fix_eph.cpp:350-429) and EPH_FDM::solve per replica
   const bool fric = sys->flags & EPH_FRICTION, rnd = sys->flags & EPH_RANDOM;
@@ -328,6 +332,57 @@ int eph_ensemble::enqueue_step(int nlog) {
     EPH_HIP(this, launch_fill_double(dT.p, G, 0.0, s));
     EPH_HIP(this, launch_nonfinite(T.p, G, nonfinite.p, s));
   }
+  return EPH_OK;
+}
+
+// Time each form of the two neighbour sweeps on the current system and keep
+// the faster.  The density sweep only rewrites its own outputs; the force
+// sweep is timed with the force application switched off, so f is untouched.
+// Either way the next step recomputes everything these launches wrote.
+int eph_ensemble::autotune(double* ms) {
+  hipStream_t s = sys->stream;
+  AtomsView a = sys->atoms_view();
+  a.nghost = ghost_cap;
+  const WorkView w = sys->work_view();
+  const int* ng = flags.p + 7;
+  StepParams p = sys->step_params();
+  p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
+  EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
+  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s, 0));
+  EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
+  StepParams pf = p;
+  pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
+  hipEvent_t e0, e1;
+  EPH_HIP(this, hipEventCreate(&e0));
+  EPH_HIP(this, hipEventCreate(&e1));
+  double t[4] = {0, 0, 0, 0};
+  const int nv = batch_lds_possible(sys->tabs) ? 2 : 1;
+  for (int phase = 0; phase < 2; ++phase) {
+    for (int v = 0; v < nv; ++v) {
+      for (int rep = 0; rep < 4; ++rep) {  // first launch warms up
+        EPH_HIP(this, hipEventRecord(e0, s));
+        if (phase == 0)
+          EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
+                                               step_dev.p, s, v));
+        else if (sys->model == 4)
+          EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p, nchunk, s, v));
+        EPH_HIP(this, hipEventRecord(e1, s));
+        EPH_HIP(this, hipEventSynchronize(e1));
+        float m = 0;
+        EPH_HIP(this, hipEventElapsedTime(&m, e0, e1));
+        if (rep > 0) t[2 * phase + v] += m / 3.0;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  var_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
+  var_force = (nv > 1 && t[3] < t[2]) ? 1 : 0;
+  if (ms) std::copy(t, t + 4, ms);
+  // a recorded step holds the old forms
+  if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
+  if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
+  graph_nlog = -1;
   return EPH_OK;
 }
 
@@ -478,6 +533,25 @@ int eph_ensemble_run(eph_ensemble* e, int nsteps, int use_graph, double* ms, int
   e->sys->nghost = fl[7];
   if (fl[1]) return e->fail(EPH_ERR_STATE, "ensemble ghost or list capacity exceeded during the run");
   if (nf & 2) return e->fail(EPH_ERR_NONFINITE, "T_e grid became non-finite (FDM instability)");
+  return EPH_OK;
+}
+
+int eph_ensemble_autotune(eph_ensemble* e, int* variants, double* ms) {
+  if (!e) return EPH_ERR_ARGS;
+  if (!e->ready) return e->fail(EPH_ERR_STATE, "ensemble not set up");
+  int rc = e->autotune(ms);
+  if (rc) return rc;
+  if (variants) { variants[0] = e->var_density; variants[1] = e->var_force; }
+  return EPH_OK;
+}
+
+int eph_ensemble_set_variants(eph_ensemble* e, int density, int force) {
+  if (!e || density < 0 || density > 1 || force < 0 || force > 1) return EPH_ERR_ARGS;
+  e->var_density = density;
+  e->var_force = force;
+  if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
+  if (e->graph) { (void)hipGraphDestroy(e->graph); e->graph = nullptr; }
+  e->graph_nlog = -1;
   return EPH_OK;
 }
 

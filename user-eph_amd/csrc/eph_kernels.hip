@@ -887,6 +887,64 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
   prl_forces_atom<FRIC, RAND, MULTI>(c.start + threadIdx.x, a, t, t.rho_r2, grids[c.rep], p, w);
 }
 
+// Persistent LDS-table forms: each 1024-thread block stages the rho(r^2)
+// tables once and walks tiles of four consecutive chunks (one chunk per 256
+// threads); the tiles of XCD x form one contiguous range.  Same per-atom
+// arithmetic as the chunk-per-block forms.
+constexpr int kChunksPerTile = kBigBlock / kBlock;
+
+template <int XI, bool FRIC, bool MULTI>
+__global__ __launch_bounds__(kBigBlock) void k_density_w_batch_lds(AtomsView a, TablesView t,
+                                                                   StepParams p, WorkView w,
+                                                                   const Chunk* __restrict__ chunks,
+                                                                   int nchunks,
+                                                                   const uint32_t* __restrict__ seeds,
+                                                                   const uint64_t* __restrict__ step) {
+  extern __shared__ double4 s_tab[];
+  const int ntab = t.n_elem * t.n_rho;
+  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  __syncthreads();
+  const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
+  int tile, stp;
+  xcd_tiles(ntiles, &tile, &stp);
+  const int end = xcd_tiles_end(ntiles);
+  StepParams pr = p;
+  pr.step = *step;
+  const int lane = static_cast<int>(threadIdx.x) % kBlock;
+  for (; tile < end; tile += stp) {
+    const int ci = tile * kChunksPerTile + static_cast<int>(threadIdx.x) / kBlock;
+    if (ci >= nchunks) continue;
+    const Chunk c = chunks[ci];
+    if (lane >= c.count) continue;
+    pr.seed = seeds[c.rep];
+    density_w_atom<XI, FRIC, MULTI>(c.start + lane, a, t, s_tab, pr, w);
+  }
+}
+
+template <bool FRIC, bool RAND, bool MULTI>
+__global__ __launch_bounds__(kBigBlock) void k_prl_forces_batch_lds(AtomsView a, TablesView t,
+                                                                    const GridView* __restrict__ grids,
+                                                                    StepParams p, WorkView w,
+                                                                    const Chunk* __restrict__ chunks,
+                                                                    int nchunks) {
+  extern __shared__ double4 s_tab[];
+  const int ntab = t.n_elem * t.n_rho;
+  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  __syncthreads();
+  const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
+  int tile, stp;
+  xcd_tiles(ntiles, &tile, &stp);
+  const int end = xcd_tiles_end(ntiles);
+  const int lane = static_cast<int>(threadIdx.x) % kBlock;
+  for (; tile < end; tile += stp) {
+    const int ci = tile * kChunksPerTile + static_cast<int>(threadIdx.x) / kBlock;
+    if (ci >= nchunks) continue;
+    const Chunk c = chunks[ci];
+    if (lane >= c.count) continue;
+    prl_forces_atom<FRIC, RAND, MULTI>(c.start + lane, a, t, s_tab, grids[c.rep], p, w);
+  }
+}
+
 template <bool FRIC, bool RAND>
 __global__ __launch_bounds__(kBlock) void k_deposit_atoms_batch(AtomsView a,
                                                                 const GridView* __restrict__ grids,
@@ -1314,11 +1372,39 @@ hipError_t launch_d4_component(const double4* in, int n, int comp, double* out, 
 
 
 // ---------------------------------------------------------- batched wrappers
+bool batch_lds_possible(const TablesView& t) {
+  const size_t bytes = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+  int dev = 0, lds_max = 0;
+  if (bytes > kLdsTableMax || hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+    return false;
+  return bytes <= static_cast<size_t>(lds_max);
+}
+
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w, const Chunk* chunks, int nchunks,
-                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s) {
+                                  const uint32_t* seeds, const uint64_t* step, hipStream_t s,
+                                  int variant) {
   if (nchunks == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
+  if (variant == 1 && batch_lds_possible(t)) {
+    const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+    const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
+#define EPH_DWBL(XI, F, M)                                                                      \
+  k_density_w_batch_lds<XI, F, M><<<lds_grid(k_density_w_batch_lds<XI, F, M>, lds, ntiles), kBigBlock, \
+                                    lds, s>>>(a, t, p, w, chunks, nchunks, seeds, step)
+#define EPH_DWBL_M(XI, F) if (multi) EPH_DWBL(XI, F, true); else EPH_DWBL(XI, F, false)
+#define EPH_DWBL_F(XI) if (fric) { EPH_DWBL_M(XI, true); } else { EPH_DWBL_M(XI, false); }
+    switch (p.xi_mode) {
+      case 1: EPH_DWBL_F(1); break;
+      case 2: EPH_DWBL_F(2); break;
+      default: EPH_DWBL_F(0); break;
+    }
+#undef EPH_DWBL_F
+#undef EPH_DWBL_M
+#undef EPH_DWBL
+    return hipGetLastError();
+  }
 #define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, 0, s>>>(a, t, p, w, chunks, seeds, step)
 #define EPH_DWB_M(XI, F) if (multi) EPH_DWB(XI, F, true); else EPH_DWB(XI, F, false)
 #define EPH_DWB_F(XI) if (fric) { EPH_DWB_M(XI, true); } else { EPH_DWB_M(XI, false); }
@@ -1335,9 +1421,23 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
 
 hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, const GridView* grids,
                                    const StepParams& p, const WorkView& w, const Chunk* chunks,
-                                   int nchunks, hipStream_t s) {
+                                   int nchunks, hipStream_t s, int variant) {
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (nchunks == 0 || (!fric && !rnd)) return hipSuccess;
+  if (variant == 1 && batch_lds_possible(t)) {
+    const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
+    const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
+#define EPH_PFBL(F, R, M)                                                                        \
+  k_prl_forces_batch_lds<F, R, M><<<lds_grid(k_prl_forces_batch_lds<F, R, M>, lds, ntiles), kBigBlock, \
+                                    lds, s>>>(a, t, grids, p, w, chunks, nchunks)
+#define EPH_PFBL_M(F, R) if (multi) EPH_PFBL(F, R, true); else EPH_PFBL(F, R, false)
+    if (fric && rnd) { EPH_PFBL_M(true, true); }
+    else if (fric) { EPH_PFBL_M(true, false); }
+    else { EPH_PFBL_M(false, true); }
+#undef EPH_PFBL_M
+#undef EPH_PFBL
+    return hipGetLastError();
+  }
 #define EPH_PFB(F, R, M) k_prl_forces_batch<F, R, M><<<nchunks, kBlock, 0, s>>>(a, t, grids, p, w, chunks)
 #define EPH_PFB_M(F, R) if (multi) EPH_PFB(F, R, true); else EPH_PFB(F, R, false)
   if (fric && rnd) { EPH_PFB_M(true, true); }

@@ -122,6 +122,8 @@ _SIG = {
     "eph_ensemble_setup": (C.c_int, [vp, C.c_longlong]),
     "eph_ensemble_run": (C.c_int, [vp, C.c_int, C.c_int, dp, ip]),
     "eph_ensemble_size": (C.c_int, [vp, ip, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]),
+    "eph_ensemble_autotune": (C.c_int, [vp, ip, dp]),
+    "eph_ensemble_set_variants": (C.c_int, [vp, C.c_int, C.c_int]),
     "eph_ensemble_info": (C.c_int, [vp, C.c_int, ip, ip, ip]),
     "eph_ensemble_get": (C.c_int, [vp, C.c_int, C.c_int, dp]),
     "eph_adjoint_create": (C.c_int, [C.POINTER(vp), vp, vp, C.c_int, C.c_int, dp]),
@@ -593,6 +595,20 @@ class Ensemble:
         self._check(lib().eph_ensemble_run(self.h, nsteps, 1 if graph else 0, C.byref(ms),
                                            _i(rebuilt) if log else None))
         return dict(ms=ms.value, rebuilt=rebuilt[:nsteps] if log else None)
+
+    def autotune(self):
+        """pick the faster form of the density and force sweeps (same bits either way)"""
+        v = np.zeros(2, np.int32)
+        ms = np.zeros(4)
+        self._check(lib().eph_ensemble_autotune(self.h, _i(v), _d(ms)))
+        names = ("chunk", "lds")
+        return dict(density=names[v[0]], force=names[v[1]],
+                    density_ms={"chunk": ms[0], "lds": ms[1]}, force_ms={"chunk": ms[2], "lds": ms[3]})
+
+    def set_variants(self, density, force):
+        """pin the sweep forms: 'chunk' or 'lds' each"""
+        f = {"chunk": 0, "lds": 1}
+        self._check(lib().eph_ensemble_set_variants(self.h, f[density], f[force]))
 
     def size(self):
         n, na, nc = C.c_int(), C.c_longlong(), C.c_longlong()
