@@ -58,17 +58,20 @@ def write_inputs(tmp, cells, a, grid, species):
     return beta, names, tin
 
 
-def cpu_baseline(args_fix, species, snap, md, dt, nsteps):
-    """The oracle (scalar C++ restatement of FixEPH) on the same system."""
+def cpu_baseline(args_fix, species, snap, md, dt, nsteps, threads, first=1):
+    """The oracle (C++ restatement of FixEPH) on the same system; its per-atom loops
+    run on `threads` host threads (OpenMP), the deposit and the FDM solve serially
+    as in the reference."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import eph_oracle as eo
     from eph_amd import inputs
+    used = eo.set_threads(threads)
     fix = eo.Fix(args_fix, species, snap.boxlo, snap.boxhi, dt, inputs.BOLTZ, inputs.FTM2V,
                  masses=[inputs.MASS_CU] * species)
     md.lj(snap)
     t_fix = 0.0
-    for s in range(1, nsteps + 1):
+    for s in range(first, first + nsteps):
         t0 = time.perf_counter()
         fix.initial_integrate(snap)
         t_fix += time.perf_counter() - t0
@@ -78,8 +81,8 @@ def cpu_baseline(args_fix, species, snap, md, dt, nsteps):
         fix.final_integrate(snap)
         fix.end_of_step(snap, s)
         t_fix += time.perf_counter() - t0
-        log(f"cpu baseline step {s}/{nsteps}: {t_fix:.1f} s")
-    return snap.nlocal * nsteps / t_fix, t_fix
+        log(f"cpu baseline ({used} threads) step {s - first + 1}/{nsteps}: {t_fix:.1f} s")
+    return snap.nlocal * nsteps / t_fix, t_fix, used
 
 
 ENS_REPLICAS, ENS_STEPS, ENS_DT = 64, 300, 2.5e-4
@@ -96,6 +99,11 @@ def write_uniform_grid(path, n, L, steps):
         f.write("NULL\n")
         f.write("\n".join(f"{i} {j} {k} 300.0 0.0 1.0 3e-06 0.1 1 0" for i, j, k in idx.tolist()))
         f.write("\n")
+
+
+def copy_snapshot(s):
+    import copy
+    return copy.deepcopy(s)
 
 
 def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
@@ -245,7 +253,8 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=80)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="one-thread CPU baseline steps")
+    ap.add_argument("--cpu-all-steps", type=int, default=6, help="all-thread CPU baseline steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a plain device copy (k_copy_probe) of MIB MiB three times, to "
@@ -288,7 +297,11 @@ def main():
     if not args.no_cpu_baseline:
         # timed before the device run, while the harness lists still match snap0
         log("cpu baseline ...")
-        cpu = cpu_baseline(fargs, species, md.snapshot(), md, dt, args.cpu_steps)
+        snap0 = md.snapshot()
+        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or
+                             (os.cpu_count() or 1)))
+        cpu1 = cpu_baseline(fargs, species, copy_snapshot(snap0), md, dt, args.cpu_steps, 1)
+        cpu = cpu_baseline(fargs, species, snap0, md, dt, args.cpu_all_steps, threads)
     md.attach(fix)
     g = fix.grid()
     T0max = g.get("T").max()
@@ -379,10 +392,14 @@ def main():
         "cpu_baseline": None,
     }
     if cpu is not None:
-        v_cpu, t_cpu = cpu
-        out["cpu_baseline"] = {"value": v_cpu, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                               "sample": f"{args.cpu_steps} fix steps of the same {N}-atom system "
-                                         f"({t_cpu:.1f} s), scalar oracle on one host core"}
+        v_cpu, t_cpu, used = cpu
+        out["cpu_baseline"] = {"value": v_cpu, "unit": "atom-steps/s", "cores": used, "kind": "port",
+                               "sample": f"{args.cpu_all_steps} fix steps of the same {N}-atom "
+                                         f"system ({t_cpu:.1f} s), the oracle's per-atom loops on "
+                                         f"{used} host threads, deposit and FDM serial"}
+        v1, t1, _ = cpu1
+        out["cpu_baseline_1core"] = {"value": v1, "unit": "atom-steps/s", "cores": 1, "kind": "port",
+                                     "sample": f"{args.cpu_steps} fix steps ({t1:.1f} s), one thread"}
     if args.calibrate_copy > 0:
         # plain copy kernel (k_copy_probe) of a known size, in this same process
         nbytes = args.calibrate_copy * (1 << 20)

@@ -31,6 +31,9 @@
 //   LAMMPS RanMars::gaussian .. NOT restated (LAMMPS absent): parity unpinned;
 //                               noise is injected instead.
 
+#include <omp.h>
+
+#include <atomic>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -184,7 +187,7 @@ struct OBeta {
   std::vector<std::string> names;
   std::vector<int> numbers;
   std::vector<OSpline> rho, rho_r_sq, beta, alpha;
-  bool warned = false;
+  std::atomic<bool> warned{false};
 
   bool load(const char* path, std::string& err) {
     std::ifstream fd(path);
@@ -222,11 +225,11 @@ struct OBeta {
   }
   double get_rho_r_sq(size_t e, double r_sq) const { return rho_r_sq[e](r_sq); }  // :164-169
   double get_beta(size_t e, double rho_i) {                                       // :171-184
-    if (rho_i > rho_cutoff) { warned = true; return 0.; }
+    if (rho_i > rho_cutoff) { warned.store(true, std::memory_order_relaxed); return 0.; }
     return beta[e](rho_i);
   }
   double get_alpha(size_t e, double rho_i) {                                      // :186-198
-    if (rho_i > rho_cutoff) { warned = true; return 0.; }
+    if (rho_i > rho_cutoff) { warned.store(true, std::memory_order_relaxed); return 0.; }
     return alpha[e](rho_i);
   }
 };
@@ -645,6 +648,7 @@ struct OFix {
   // fix_eph.cpp:431-466
   void calculate_environment(int nlocal, const double* x, const int* type, const int* mask,
                              const int* off, const int* nbr) {
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < nlocal; ++i) {
       rho_i[i] = 0;
       if (mask[i] & groupbit) {
@@ -663,6 +667,7 @@ struct OFix {
   void force_prl(int nlocal, int nghost, const double* x, const double* v, const int* type,
                  const int* mask, const int* off, const int* nbr, const int* owner) {
     if (eph_flag & FRICTION) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nlocal; ++i) {
         if (!(mask[i] & groupbit)) continue;
         int itype = type[i];
@@ -687,6 +692,7 @@ struct OFix {
         }
       }
       forward(w_i, 3, nlocal, nghost, owner);  // state WI, :743-744
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nlocal; ++i) {
         if (!(mask[i] & groupbit)) continue;
         int itype = type[i];
@@ -713,6 +719,7 @@ struct OFix {
       }
     }
     if (eph_flag & RANDOM) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nlocal; ++i) {
         if (!(mask[i] & groupbit)) continue;
         int itype = type[i];
@@ -756,6 +763,7 @@ struct OFix {
     std::fill_n(f_EPH.begin(), 3 * nlocal, 0.0);
     std::fill_n(f_RNG.begin(), 3 * nlocal, 0.0);
     if (eph_flag & RANDOM) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < nlocal; ++i)
         if (mask[i] & groupbit) {
           if (xi_in) {
@@ -1002,6 +1010,13 @@ void ora_fix_get(void* h, int which, int n, double* out) {
 }
 void* ora_fix_fdm(void* h) { return &static_cast<OFix*>(h)->fdm; }
 int ora_fix_post_run(void* h) { auto* f = static_cast<OFix*>(h); return f->fdm.save_state(f->T_state.c_str()) ? 0 : -1; }
+// threads of the per-atom loops (the deposit and the FDM solve stay serial,
+// as in the reference); returns the count in effect
+int ora_set_threads(int n) {
+  if (n > 0) omp_set_num_threads(n);
+  return omp_get_max_threads();
+}
+
 double ora_fix_alpha(void* h, int elem, double rho) { return static_cast<OFix*>(h)->beta.get_alpha(elem, rho); }
 double ora_fix_rho_r_sq(void* h, int elem, double r2) { return static_cast<OFix*>(h)->beta.get_rho_r_sq(elem, r2); }
 

@@ -86,6 +86,7 @@ def lib():
             "ora_fix_fdm": (C.c_void_p, [C.c_void_p]),
             "ora_fix_post_run": (C.c_int, [C.c_void_p]),
             "ora_fix_alpha": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_set_threads": (C.c_int, [C.c_int]),
             "ora_fix_rho_r_sq": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
         }
         for name, (res, args) in sig.items():
@@ -321,3 +322,8 @@ def ref_available():
 def ref_run(*args):
     out = subprocess.run([REF_DRIVER, *map(str, args)], check=True, capture_output=True, text=True)
     return out.stdout
+
+
+def set_threads(n):
+    """threads of the per-atom loops (0: leave as is); returns the count in effect"""
+    return lib().ora_set_threads(int(n))
