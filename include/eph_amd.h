@@ -148,6 +148,17 @@ int eph_sync(eph_context* ctx);
 /* the context's HIP stream (hipStream_t), for timing with HIP events */
 void* eph_stream(eph_context* ctx);
 
+/* Kernel forms of the two neighbour sweeps (no counterpart in the reference;
+ * every form gives the same bits).  0: one 256-thread block per 256 atoms,
+ * spline tables read through the caches; 1: persistent 1024-thread blocks
+ * with the rho(r^2) tables staged in LDS (only when they fit); -1: chosen by
+ * system size (the default).  eph_autotune times both forms of each sweep on
+ * the resident atoms (after eph_dev_post_force or the harness's attach) and
+ * keeps the faster; forms[2] receives the choice (density, force) and ms[4]
+ * the times (density form 0, 1, force form 0, 1).  f is not changed. */
+int eph_set_sweep_forms(eph_context* ctx, int density, int force);
+int eph_autotune(eph_context* ctx, int* forms, double* ms);
+
 /* Per-kernel timing with HIP events on the context's stream (no counterpart
  * in the reference; used to measure kernel durations).  eph_timing_report with
  * k < 0 returns the number of kernels recorded; with k >= 0 it fills the name,

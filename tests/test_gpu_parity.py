@@ -80,20 +80,44 @@ def test_trajectory_fcc10_100_steps(oracle, tmp_path):
     assert abs(p.gpu.compute_vector(1) - p.ora.compute_vector(1)) <= 1e-12 * p.ora.compute_vector(1)
 
 
-def test_alloy_cascade_small(oracle, tmp_path):
-    """4 species, 1 keV PKA, hot spot, dt 0.1 fs: multi-element tables and rebuilds."""
+@pytest.mark.parametrize("forms", [("auto", "auto"), ("lds", "lds")])
+def test_alloy_cascade_small(oracle, tmp_path, forms):
+    """4 species, 1 keV PKA, hot spot, dt 0.1 fs: multi-element tables and
+    rebuilds; by default (chunk forms at this size) and with the LDS-table
+    forms forced on a small system."""
     p = Pair(oracle, ea, str(tmp_path), cells=8, species=4, T=300.0, pka_eV=1000.0, grid=8,
              dt=1e-4, hot_spot=(3000.0, 6.0))
+    p.gpu.set_sweep_forms(*forms)
     for _ in range(30):
         p.step()
     _fix_state_close(p)
 
 
-def test_alloy_lds_table_path(oracle, tmp_path):
-    """fcc 21^3 (37,044 atoms), 4 species: large enough for the persistent
-    kernels that stage all four rho(r^2) tables (128 KB) in LDS."""
+@pytest.mark.parametrize("forms", [("auto", "auto"), ("chunk", "chunk"), ("chunk", "lds"),
+                                   ("lds", "chunk")])
+def test_alloy_lds_table_path(oracle, tmp_path, forms):
+    """fcc 21^3 (37,044 atoms), 4 species: large enough that the default is the
+    persistent kernels staging all four rho(r^2) tables (128 KB) in LDS; every
+    combination of sweep forms gives the oracle's bits."""
     p = Pair(oracle, ea, str(tmp_path), cells=21, species=4, T=900.0, disp=0.05, grid=7, dt=5e-4)
+    p.gpu.set_sweep_forms(*forms)
     for _ in range(3):
+        p.step()
+    _fix_state_close(p)
+
+
+def test_autotune_keeps_forces_and_bits(oracle, tmp_path):
+    """eph_autotune between steps changes neither f nor the trajectory"""
+    p = Pair(oracle, ea, str(tmp_path), cells=21, species=4, T=900.0, disp=0.05, grid=7, dt=5e-4)
+    p.step()
+    before = p.gpu.dev_atoms()
+    tuned = p.gpu.autotune()
+    after = p.gpu.dev_atoms()
+    for a, b in zip(before, after):
+        assert np.array_equal(a, b)
+    assert tuned["density"] in ("chunk", "lds") and tuned["force"] in ("chunk", "lds")
+    assert all(v > 0 for v in tuned["density_ms"].values()), tuned
+    for _ in range(2):
         p.step()
     _fix_state_close(p)
 

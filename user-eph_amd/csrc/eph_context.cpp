@@ -401,18 +401,64 @@ int eph_context::dev_post_force() {
   EPH_HIP(this, launch_pack_pv(a, w, stream));
   timed_end("pack_pv", e);
   timed_begin("density_w", &e);
-  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream));
+  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
   timed_end("density_w", e);
   timed_begin("ghost_rec", &e);
   EPH_HIP(this, launch_ghost_rec(a, w, stream));
   timed_end("ghost_rec", e);
   if (model == 4) {
     timed_begin("prl_forces", &e);
-    EPH_HIP(this, launch_prl_forces(a, tabs, grid->v, p, w, stream));
+    EPH_HIP(this, launch_prl_forces(a, tabs, grid->v, p, w, stream, form_force));
     timed_end("prl_forces", e);
   }
   xi_mode_next = 0;
   have_forces = true;
+  return EPH_OK;
+}
+
+// Times both forms of each sweep on the resident atoms and keeps the faster.
+// The density sweep rewrites only what the next post_force rewrites anyway;
+// the force sweep runs with both force additions off, so f is not touched.
+int eph_context::autotune(int* forms, double* ms) {
+  if (!have_atoms || !have_list) return fail(EPH_ERR_STATE, "atoms and neighbour list must be set");
+  StepParams p = step_params();
+  p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
+  const AtomsView a = atoms_view();
+  const WorkView w = work_view();
+  EPH_HIP(this, launch_pack_pv(a, w, stream));
+  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
+  EPH_HIP(this, launch_ghost_rec(a, w, stream));
+  StepParams pf = p;
+  pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  EPH_HIP(this, hipEventCreate(&e0));
+  EPH_HIP(this, hipEventCreate(&e1));
+  double t[4] = {0, 0, 0, 0};
+  const int nv = lds_tables_possible(tabs) ? 2 : 1;
+  hipError_t he = hipSuccess;
+  for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
+    if (phase == 1 && model != 4) break;
+    for (int v = 0; v < nv && he == hipSuccess; ++v) {
+      for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // the first launch warms up
+        he = hipEventRecord(e0, stream);
+        if (he == hipSuccess)
+          he = phase == 0 ? launch_density_w(a, tabs, p, w, stream, v)
+                          : launch_prl_forces(a, tabs, grid->v, pf, w, stream, v);
+        if (he == hipSuccess) he = hipEventRecord(e1, stream);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
+        float m = 0;
+        if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
+        if (rep > 0) t[2 * phase + v] += m / 3.0;
+      }
+    }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (he != hipSuccess) return fail_hip(he, "autotune");
+  form_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
+  form_force = (nv > 1 && model == 4 && t[3] < t[2]) ? 1 : 0;
+  if (forms) { forms[0] = form_density; forms[1] = form_force; }
+  if (ms) std::copy(t, t + 4, ms);
   return EPH_OK;
 }
 
@@ -876,6 +922,19 @@ int eph_sync(eph_context* ctx) {
 }
 
 void* eph_stream(eph_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
+
+int eph_set_sweep_forms(eph_context* ctx, int density, int force) {
+  if (!ctx || density < -1 || density > 1 || force < -1 || force > 1) return EPH_ERR_ARGS;
+  if ((density == 1 || force == 1) && !lds_tables_possible(ctx->tabs))
+    return ctx->fail(EPH_ERR_ARGS, "the spline tables do not fit in LDS on this device");
+  ctx->form_density = density;
+  ctx->form_force = force;
+  return EPH_OK;
+}
+
+int eph_autotune(eph_context* ctx, int* forms, double* ms) {
+  return ctx ? ctx->autotune(forms, ms) : EPH_ERR_ARGS;
+}
 
 int eph_timing_enable(eph_context* ctx, int on) {
   if (!ctx) return EPH_ERR_ARGS;

@@ -132,6 +132,8 @@ struct eph_context {
       flags_dev;
   std::vector<int> type_host;
   int xi_mode_next = 0;  // 1 once eph_set_noise injected values
+  // kernel forms of the two sweeps (-1 chosen by system size; see eph_autotune)
+  int form_density = -1, form_force = -1;
   long long step = 0;
   double* pinned = nullptr;  // small pinned host scratch
   // optional per-kernel timing
@@ -160,6 +162,7 @@ struct eph_context {
   int dev_final_integrate();
   int dev_end_of_step(long long ntimestep);
   int dev_deposit();
+  int autotune(int* forms, double* ms);
   int check_flags();
   // timing helpers
   hipEvent_t take_event();

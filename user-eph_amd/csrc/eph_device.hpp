@@ -137,12 +137,16 @@ struct Chunk {
 hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);
 hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
                              int* nn, int stride, hipStream_t s);
+// form: 0 one block per 256 atoms with tables read from global memory,
+// 1 persistent blocks with the tables in LDS (when they fit), -1 by size
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
-                            const WorkView& w, hipStream_t s);
+                            const WorkView& w, hipStream_t s, int form = -1);
+// the tables fit in one workgroup's LDS on the current device
+bool lds_tables_possible(const TablesView& t);
 hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s);
 hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s);
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
-                             const StepParams& p, const WorkView& w, hipStream_t s);
+                             const StepParams& p, const WorkView& w, hipStream_t s, int form = -1);
 hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 hipError_t launch_fill_double(double* p, long n, double v, hipStream_t s);

@@ -41,7 +41,7 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __
 // r^2 < rc^2 into an active list, which the w sweep and the force sweep
 // re-read in the same order (they skip exactly the other entries).
 // `rtab` holds the rho(r^2) tables, in global memory or staged in LDS.
-template <int XI, bool FRIC, bool MULTI>
+template <int XI, bool FRIC, bool MULTI, bool X2>
 __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, const TablesView& t,
                                                const double4* __restrict__ rtab,
                                                const StepParams& p, const WorkView& w) {
@@ -50,7 +50,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   const bool grp = (a.mask[i] & a.groupbit) != 0;
   double rho = 0.0;
   int na = 0;
-  if (grp) {
+  if (grp && !X2) {
     // each neighbour's position is loaded one iteration ahead of its use
     const int nn = w.nnbr[i];
     int jn = nn > 0 ? w.nbr[i] : 0;
@@ -68,6 +68,44 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
         const int ej = MULTI ? static_cast<int>(pj.w) : 0;
         rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
         w.act[na * stride + i] = j;
+        ++na;
+      }
+    }
+  }
+  if (grp && X2) {
+    // Two entries per step: their r^2 and table values are independent, and
+    // are added to rho in list order.  The positions of the next two entries
+    // and the indices of the two after them are in flight meanwhile.
+    const int nn = w.nnbr[i];
+    const int* __restrict__ L = w.nbr + i;
+    int ja = nn > 0 ? L[0] : 0, jb = nn > 1 ? L[stride] : 0;
+    double4 qa = nn > 0 ? a.pos[ja] : pi, qb = nn > 1 ? a.pos[jb] : pi;
+    int ka = nn > 2 ? L[2 * stride] : 0, kb = nn > 3 ? L[3 * stride] : 0;
+    for (int k = 0; k < nn; k += 2) {
+      const int j0 = ja, j1 = jb;
+      const double4 p0 = qa, p1 = qb;
+      ja = ka; jb = kb;
+      if (k + 2 < nn) qa = a.pos[ka];
+      if (k + 3 < nn) qb = a.pos[kb];
+      if (k + 4 < nn) ka = L[(k + 4) * stride];
+      if (k + 5 < nn) kb = L[(k + 5) * stride];
+      const double ax = p0.x - pi.x, ay = p0.y - pi.y, az = p0.z - pi.z;
+      const double bx = p1.x - pi.x, by = p1.y - pi.y, bz = p1.z - pi.z;
+      const double r2a = ax * ax + ay * ay + az * az;
+      const double r2b = bx * bx + by * by + bz * bz;
+      const bool ua = r2a < t.rc2, ub = k + 1 < nn && r2b < t.rc2;
+      const int ea = MULTI ? static_cast<int>(p0.w) : 0;
+      const int eb = MULTI ? static_cast<int>(p1.w) : 0;
+      const double ta = cubic_at(rtab + ea * t.n_rho, t.n_rho, t.inv_dr2, r2a);
+      const double tb = cubic_at(rtab + eb * t.n_rho, t.n_rho, t.inv_dr2, r2b);
+      if (ua) {
+        rho += ta;
+        w.act[na * stride + i] = j0;
+        ++na;
+      }
+      if (ub) {
+        rho += tb;
+        w.act[na * stride + i] = j1;
         ++na;
       }
     }
@@ -133,7 +171,7 @@ template <int XI, bool FRIC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
                                                       WorkView w) {
   const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI>(i, a, t, t.rho_r2, p, w);
+  if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI, true>(i, a, t, t.rho_r2, p, w);
 }
 
 // Persistent form: each block stages the rho(r^2) tables in LDS once and then
@@ -169,7 +207,7 @@ __global__ __launch_bounds__(kBigBlock) void k_density_w_lds(AtomsView a, Tables
   const int end = xcd_tiles_end(ntiles);
   for (; tile < end; tile += step) {
     const int i = tile * kBigBlock + threadIdx.x;
-    if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI>(i, a, t, s_tab, p, w);
+    if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI, false>(i, a, t, s_tab, p, w);
   }
 }
 
@@ -202,7 +240,11 @@ __global__ __launch_bounds__(kBlock) void k_pack_pv(AtomsView a, WorkView w, int
 // fix_eph.cpp:748-787 (friction) and :791-836 (random), then the force
 // application of :892-906.  Both sums share the neighbour gather; each
 // accumulates in list order exactly as its reference loop does.
-template <bool FRIC, bool RAND, bool MULTI>
+// Each neighbour's record is loaded one iteration ahead of its use; with DEEP
+// its index is loaded two ahead, so the record load never waits for an index
+// load issued in the same iteration (more registers: the LDS-table forms,
+// whose occupancy is set by LDS, use it).
+template <bool FRIC, bool RAND, bool MULTI, bool DEEP = false>
 __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a, const TablesView& t,
                                                 const double4* __restrict__ rtab,
                                                 const GridView& g, const StepParams& p,
@@ -218,17 +260,23 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
     const double alpha_i = xai.w;
     const int ei = MULTI ? static_cast<int>(pi.w) : 0;
     const int na = w.nact[i];
-    // each neighbour's record is loaded one iteration ahead of its use
     int jn = na > 0 ? w.act[i] : 0;
     double4 nrw = rwi, np = pi, nxa = xai;
     if (na > 0) {
       nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
     }
+    if (DEEP) jn = na > 1 ? w.act[stride + i] : 0;  // index of entry k + 1
     for (int k = 0; k < na; ++k) {
       const double4 rwj = nrw, pj = np, xaj = nxa;
       if (k + 1 < na) {
-        jn = w.act[(k + 1) * stride + i];
-        nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+        if (DEEP) {
+          const int jc = jn;
+          if (k + 2 < na) jn = w.act[(k + 2) * stride + i];
+          nrw = w.rec[jc].rw; np = w.rec[jc].p; nxa = w.rec[jc].xa;
+        } else {
+          jn = w.act[(k + 1) * stride + i];
+          nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+        }
       }
       const double rho_j = rwj.w;
       if (!(rho_j > 0.0)) continue;
@@ -288,7 +336,7 @@ template <bool FRIC, bool RAND, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
                                                        StepParams p, WorkView w) {
   const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI>(i, a, t, t.rho_r2, g, p, w);
+  if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, t.rho_r2, g, p, w);
 }
 
 template <bool FRIC, bool RAND, bool MULTI>
@@ -304,7 +352,7 @@ __global__ __launch_bounds__(kBigBlock) void k_prl_forces_lds(AtomsView a, Table
   const int end = xcd_tiles_end(ntiles);
   for (; tile < end; tile += step) {
     const int i = tile * kBigBlock + threadIdx.x;
-    if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI>(i, a, t, s_tab, g, p, w);
+    if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, s_tab, g, p, w);
   }
 }
 
@@ -861,7 +909,7 @@ __global__ __launch_bounds__(kBlock) void k_density_w_batch(AtomsView a, TablesV
   StepParams pr = p;
   pr.seed = seeds[c.rep];
   pr.step = *step;
-  density_w_atom<XI, FRIC, MULTI>(c.start + threadIdx.x, a, t, t.rho_r2, pr, w);
+  density_w_atom<XI, FRIC, MULTI, true>(c.start + threadIdx.x, a, t, t.rho_r2, pr, w);
 }
 
 // start of a recorded step: step += 1
@@ -884,7 +932,7 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
                                                              const Chunk* __restrict__ chunks) {
   const Chunk c = chunks[blockIdx.x];
   if (static_cast<int>(threadIdx.x) >= c.count) return;
-  prl_forces_atom<FRIC, RAND, MULTI>(c.start + threadIdx.x, a, t, t.rho_r2, grids[c.rep], p, w);
+  prl_forces_atom<FRIC, RAND, MULTI, true>(c.start + threadIdx.x, a, t, t.rho_r2, grids[c.rep], p, w);
 }
 
 // Persistent LDS-table forms: each 1024-thread block stages the rho(r^2)
@@ -917,7 +965,7 @@ __global__ __launch_bounds__(kBigBlock) void k_density_w_batch_lds(AtomsView a, 
     const Chunk c = chunks[ci];
     if (lane >= c.count) continue;
     pr.seed = seeds[c.rep];
-    density_w_atom<XI, FRIC, MULTI>(c.start + lane, a, t, s_tab, pr, w);
+    density_w_atom<XI, FRIC, MULTI, false>(c.start + lane, a, t, s_tab, pr, w);
   }
 }
 
@@ -941,7 +989,7 @@ __global__ __launch_bounds__(kBigBlock) void k_prl_forces_batch_lds(AtomsView a,
     if (ci >= nchunks) continue;
     const Chunk c = chunks[ci];
     if (lane >= c.count) continue;
-    prl_forces_atom<FRIC, RAND, MULTI>(c.start + lane, a, t, s_tab, grids[c.rep], p, w);
+    prl_forces_atom<FRIC, RAND, MULTI, true>(c.start + lane, a, t, s_tab, grids[c.rep], p, w);
   }
 }
 
@@ -1111,9 +1159,9 @@ static int lds_grid(K kernel, size_t lds_bytes, int ntiles) {
 
 // The tables go to LDS only when the device reports room for them in one
 // workgroup (gfx950: 160 KB); otherwise the sweeps read them from global memory.
-static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
+static bool lds_tables_fit(const TablesView& t) {
   const size_t bytes = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-  if (a.nlocal < kLdsMinAtoms || bytes > kLdsTableMax) return false;
+  if (bytes > kLdsTableMax) return false;
   int dev = 0, lds_max = 0;
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
@@ -1121,12 +1169,19 @@ static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
   return bytes <= static_cast<size_t>(lds_max);
 }
 
+// the default choice when no form is set: LDS tables for large systems
+static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
+  return a.nlocal >= kLdsMinAtoms && lds_tables_fit(t);
+}
+
+bool lds_tables_possible(const TablesView& t) { return lds_tables_fit(t); }
+
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
-                            const WorkView& w, hipStream_t s) {
+                            const WorkView& w, hipStream_t s, int form) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
-  if (use_lds_tables(a, t)) {
+  if (form != 0 && lds_tables_fit(t) && (form == 1 || use_lds_tables(a, t))) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
 #define EPH_DWL(XI, F, M)                                                                  \
@@ -1173,11 +1228,11 @@ hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s
 }
 
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
-                             const StepParams& p, const WorkView& w, hipStream_t s) {
+                             const StepParams& p, const WorkView& w, hipStream_t s, int form) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (!fric && !rnd) return hipSuccess;
-  if (use_lds_tables(a, t)) {
+  if (form != 0 && lds_tables_fit(t) && (form == 1 || use_lds_tables(a, t))) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
 #define EPH_PFL(F, R, M)                                                                 \

@@ -77,6 +77,8 @@ _SIG = {
     "eph_sync": (C.c_int, [vp]),
     "eph_stream": (vp, [vp]),
     "eph_timing_enable": (C.c_int, [vp, C.c_int]),
+    "eph_set_sweep_forms": (C.c_int, [vp, C.c_int, C.c_int]),
+    "eph_autotune": (C.c_int, [vp, ip, dp]),
     "eph_timing_report": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, dp, C.POINTER(C.c_longlong)]),
     "eph_copy_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, dp]),
     "eph_spline_build": (C.c_int, [C.c_double, dp, C.c_int, dp]),
@@ -336,6 +338,22 @@ class FixEPH:
         return x, v, f
 
     def timing(self, on=True): self._check(lib().eph_timing_enable(self.h, 1 if on else 0))
+
+    _FORMS = {"chunk": 0, "lds": 1, "auto": -1}
+
+    def set_sweep_forms(self, density="auto", force="auto"):
+        """kernel forms of the density and force sweeps: 'chunk', 'lds' or 'auto'"""
+        self._check(lib().eph_set_sweep_forms(self.h, self._FORMS[density], self._FORMS[force]))
+
+    def autotune(self):
+        """time both forms of each sweep on the resident atoms, keep the faster"""
+        forms = (C.c_int * 2)()
+        ms = (C.c_double * 4)()
+        self._check(lib().eph_autotune(self.h, forms, ms))
+        name = {0: "chunk", 1: "lds"}
+        return {"density": name[forms[0]], "force": name[forms[1]],
+                "density_ms": {"chunk": ms[0], "lds": ms[1]},
+                "force_ms": {"chunk": ms[2], "lds": ms[3]}}
 
     def timing_report(self):
         L = lib()
