@@ -59,7 +59,14 @@ enum eph_hook {
  * exactly as LAMMPS passes them (arg[0] = fix ID ... arg[17+] = element per
  * type).  boxlo/boxhi = domain->boxlo/boxhi, dt = update->dt, boltz and
  * ftm2v = force->boltz / force->ftm2v, ntypes = atom->ntypes.  device = HIP
- * device ordinal. */
+ * device ordinal.
+ * arg[2] selects the style: "eph" (models 1 TTM, 2 PRB, 4 PRL, -1 TESTING in
+ * arg[5]; model 3 is rejected: the reference indexes its rho table by
+ * jtype - i, fix_eph.cpp:600), "eph/coloured" (fix_eph_coloured.cpp: PRL),
+ * "eph/coloured/exp" (fix_eph_coloured_exp.cpp: arg[5] = tau0, the applied
+ * f_EPH and f_RNG are exponentially filtered) and "eph/coloured/exp/v1"
+ * (fix_eph_coloured_exp_v1.cpp: arg[5] = tau0, the w sum sees filtered
+ * velocities and the random sum filtered noise). */
 int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
                const double boxlo[3], const double boxhi[3], double dt, double boltz,
                double ftm2v, int device);
@@ -73,8 +80,15 @@ const char* eph_last_error(const eph_context* ctx);
 int eph_init(eph_context* ctx, int dimension, int nonperiodic, int triclinic, double* cutoff);
 /* FixEPH::setmask, fix_eph.cpp:293-302: returns the eph_hook bits */
 int eph_setmask(void);
-/* FixEPH::reset_dt, fix_eph.cpp:909-916 */
+/* FixEPH::reset_dt, fix_eph.cpp:909-916 (and zeta, fix_eph_coloured_exp.cpp:683-690) */
 int eph_reset_dt(eph_context* ctx, double dt);
+/* Filter state of the coloured styles, [nlocal][3] by local index: which 0 is
+ * f_dis (eph/coloured/exp) or zv (.../v1), which 1 is f_sto or zi.  The
+ * reference carries these with the atoms (pack_exchange / copy_arrays of
+ * fix_eph_coloured_exp.cpp); a caller that reorders atoms reads them before
+ * and writes them back after eph_set_atoms. */
+int eph_get_filter_state(eph_context* ctx, int which, double* out);
+int eph_set_filter_state(eph_context* ctx, int which, const double* in);
 /* atom->mass[1..ntypes] (element 0 unused, as in LAMMPS) */
 int eph_set_masses(eph_context* ctx, const double* mass, int ntypes);
 /* Fix::groupbit; atoms with (mask & groupbit) belong to the fix group */

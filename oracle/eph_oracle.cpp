@@ -21,6 +21,14 @@
 //                               Fourier-mode decay factor, energy conservation).
 //   linear table .............. compiled reference (bitwise); closed forms.
 //   calculate_environment ..... closed-form lattice density; O(N^2) brute force.
+//   force_ttm / force_prb ..... closed forms: f_EPH = -beta(rho_i) v_i (TTM);
+//                               PRB at uniform velocity (v_j = v_i) on a
+//                               lattice: f_EPH = beta(rho_i) v_i (1 - rho_r
+//                               sum / rho_i) with the rho(r) spline sum.
+//   coloured styles ........... reduce bit for bit to plain PRL when
+//                               zeta = 1 (tau0 << dt); the exponential
+//                               filter recurrence checked against plain-PRL
+//                               forces on the same trajectory.
 //   force_prl ................. dense 3N x 3N W-matrix brute force
 //                               (f_EPH = -W^T W v, f_RNG = eta sqrt(T) W^T xi),
 //                               dimer closed form, momentum / dissipation /
@@ -543,8 +551,18 @@ void ora_xi(uint32_t seed, uint32_t tag, uint64_t step, double out[3]) {
 // ---------------------------------------------------------------------------
 enum Flag { FRICTION = 0x01, RANDOM = 0x02, FDM = 0x04, NOINT = 0x08, NOFRICTION = 0x10, NORANDOM = 0x20 };
 
+// Fix styles sharing FixEPH's command layout: eph (fix_eph.cpp), eph/coloured
+// (fix_eph_coloured.cpp: PRL whatever arg[5] says), eph/coloured/exp
+// (fix_eph_coloured_exp.cpp: arg[5] = tau0, exponential filters on f_EPH and
+// f_RNG) and eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp: filters on the
+// velocities of the w sum and on the noise).
+enum Style { STYLE_EPH = 0, STYLE_COLOURED = 1, STYLE_COLOURED_EXP = 2, STYLE_COLOURED_EXP_V1 = 3 };
+
 struct OFix {
   std::string err;
+  int style = STYLE_EPH;
+  double tau0 = 0.0, zeta = 0.0;
+  std::vector<double> f_dis, f_sto, zv, zi;  // filter states, [nall][3], zero at start
   int seed = 0, eph_flag = 0, eph_model = 0, types = 0, T_freq = 0;
   std::vector<int> type_map;
   OBeta beta;
@@ -562,9 +580,21 @@ struct OFix {
   bool create(int narg, const char* const* arg, int ntypes, const double* boxlo,
               const double* boxhi, double dt, double boltz_, double ftm2v_) {
     if (narg < 18) { err = "Illegal fix eph command: too few arguments"; return false; }
+    const std::string name = arg[2];
+    if (name == "eph" || name == "eph/gpu") style = STYLE_EPH;
+    else if (name == "eph/coloured") style = STYLE_COLOURED;
+    else if (name == "eph/coloured/exp") style = STYLE_COLOURED_EXP;
+    else if (name == "eph/coloured/exp/v1") style = STYLE_COLOURED_EXP_V1;
+    else { err = "unknown fix style " + name; return false; }
     seed = std::atoi(arg[3]);
     eph_flag = (int)std::strtol(arg[4], nullptr, 0);
-    eph_model = std::atoi(arg[5]);
+    if (style >= STYLE_COLOURED_EXP) {
+      tau0 = std::atof(arg[5]);  // fix_eph_coloured_exp.cpp:113
+      eph_model = 4;
+    } else {
+      eph_model = std::atoi(arg[5]);
+      if (style == STYLE_COLOURED) eph_model = 4;  // fix_eph_coloured.cpp:652 calls force_prl()
+    }
     double v_rho = std::atof(arg[6]), v_Ce = std::atof(arg[7]);
     double v_kappa = std::atof(arg[8]), v_Te = std::atof(arg[9]);
     int nx = std::atoi(arg[10]), ny = std::atoi(arg[11]), nz = std::atoi(arg[12]);
@@ -595,8 +625,9 @@ struct OFix {
     mass.assign(types + 1, 1.0);
     return true;
   }
-  void reset_dt(double dt) {  // fix_eph.cpp:909-916
+  void reset_dt(double dt) {  // fix_eph.cpp:909-916, fix_eph_coloured_exp.cpp:683-690
     eta_factor = std::sqrt(2.0 * boltz / dt);
+    if (style >= STYLE_COLOURED_EXP) zeta = 1.0 - std::exp(-dt / tau0);
     dtv = dt;
     dtf = 0.5 * dt * ftm2v;
     fdm.dt = dt;
@@ -605,7 +636,16 @@ struct OFix {
     if (rho_i.size() < nall) {
       rho_i.resize(nall, 0.0); w_i.resize(3 * nall, 0.0); xi_i.resize(3 * nall, 0.0);
       f_EPH.resize(3 * nall, 0.0); f_RNG.resize(3 * nall, 0.0); array.resize(8 * nall, 0.0);
+      f_dis.resize(3 * nall, 0.0); f_sto.resize(3 * nall, 0.0);
+      zv.resize(3 * nall, 0.0); zi.resize(3 * nall, 0.0);
     }
+  }
+  // exponential filter z <- z (1 - zeta) + zeta u, per component
+  // (fix_eph_coloured_exp.cpp:562-564, fix_eph_coloured_exp_v1.cpp:480-482)
+  void filter(double* z, const double* u) const {
+    z[0] = z[0] * (1. - zeta) + zeta * u[0];
+    z[1] = z[1] * (1. - zeta) + zeta * u[1];
+    z[2] = z[2] * (1. - zeta) + zeta * u[2];
   }
   // fix_eph.cpp:305-328
   void initial_integrate(int nlocal, double* x, double* v, const double* f, const int* type,
@@ -664,8 +704,18 @@ struct OFix {
   }
 
   // fix_eph.cpp:687-837
-  void force_prl(int nlocal, int nghost, const double* x, const double* v, const int* type,
+  void force_prl(int nlocal, int nghost, const double* x, const double* v_in, const int* type,
                  const int* mask, const int* off, const int* nbr, const int* owner) {
+    const double* v = v_in;
+    const double* xi = xi_i.data();
+    if (style == STYLE_COLOURED_EXP_V1) {
+      if (eph_flag & FRICTION) {  // fix_eph_coloured_exp_v1.cpp:477-487
+        for (int i = 0; i < nlocal; ++i) filter(&zv[3 * i], &v_in[3 * i]);
+        forward(zv, 3, nlocal, nghost, owner);
+      }
+      v = zv.data();   // the w sum uses the filtered velocities (:514-517)
+      xi = zi.data();  // the random sum uses the filtered noise (:600-605)
+    }
     if (eph_flag & FRICTION) {
 #pragma omp parallel for schedule(static)
       for (int i = 0; i < nlocal; ++i) {
@@ -716,6 +766,10 @@ struct OFix {
           f_EPH[3 * i + 1] -= dvar * e_ij[1];
           f_EPH[3 * i + 2] -= dvar * e_ij[2];
         }
+        if (style == STYLE_COLOURED_EXP) {  // fix_eph_coloured_exp.cpp:562-568
+          filter(&f_dis[3 * i], &f_EPH[3 * i]);
+          for (int c = 0; c < 3; ++c) f_EPH[3 * i + c] = f_dis[3 * i + c];
+        }
       }
     }
     if (eph_flag & RANDOM) {
@@ -733,10 +787,10 @@ struct OFix {
           if ((e_r_sq >= r_cutoff_sq) || !(rho_i[jj] > 0)) continue;
           double alpha_j = beta.get_alpha(type_map[jtype - 1], rho_i[jj]);
           double v_rho_ji = beta.get_rho_r_sq(type_map[jtype - 1], e_r_sq);
-          double e_v_xi1 = dot(e_ij, &xi_i[3 * i]);
+          double e_v_xi1 = dot(e_ij, &xi[3 * i]);
           double var1 = alpha_i * v_rho_ji * e_v_xi1 / (rho_i[i] * e_r_sq);
           double v_rho_ij = beta.get_rho_r_sq(type_map[itype - 1], e_r_sq);
-          double e_v_xi2 = dot(e_ij, &xi_i[3 * jj]);
+          double e_v_xi2 = dot(e_ij, &xi[3 * jj]);
           double var2 = alpha_j * v_rho_ij * e_v_xi2 / (rho_i[jj] * e_r_sq);
           double dvar = var1 - var2;
           f_RNG[3 * i + 0] += dvar * e_ij[0];
@@ -748,8 +802,70 @@ struct OFix {
         f_RNG[3 * i + 0] *= var;
         f_RNG[3 * i + 1] *= var;
         f_RNG[3 * i + 2] *= var;
+        if (style == STYLE_COLOURED_EXP) {  // fix_eph_coloured_exp.cpp:619-625
+          filter(&f_sto[3 * i], &f_RNG[3 * i]);
+          for (int c = 0; c < 3; ++c) f_RNG[3 * i + c] = f_sto[3 * i + c];
+        }
       }
     }
+  }
+
+  // fix_eph.cpp:468-503 (model 1)
+  void force_ttm(int nlocal, const double* x, const double* v, const int* type, const int* mask) {
+    if (eph_flag & FRICTION)
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit) {
+          int itype = type[i];
+          double var = -beta.get_beta(type_map[itype - 1], rho_i[i]);
+          f_EPH[3 * i + 0] = var * v[3 * i + 0];
+          f_EPH[3 * i + 1] = var * v[3 * i + 1];
+          f_EPH[3 * i + 2] = var * v[3 * i + 2];
+        }
+    random_per_atom(nlocal, x, type, mask);
+  }
+  // fix_eph.cpp:489-502 and :554-567: the random force of models 1 and 2
+  void random_per_atom(int nlocal, const double* x, const int* type, const int* mask) {
+    if (eph_flag & RANDOM)
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit) {
+          int itype = type[i];
+          double v_Te = fdm.get_T(x[3 * i + 0], x[3 * i + 1], x[3 * i + 2]);
+          double var = eta_factor * beta.get_alpha(type_map[itype - 1], rho_i[i]) * std::sqrt(v_Te);
+          f_RNG[3 * i + 0] = var * xi_i[3 * i + 0];
+          f_RNG[3 * i + 1] = var * xi_i[3 * i + 1];
+          f_RNG[3 * i + 2] = var * xi_i[3 * i + 2];
+        }
+  }
+  // fix_eph.cpp:505-568 (model 2).  rho(r) of element jtype - 1, unmapped,
+  // evaluated at sqrt(r^2) (:537); no neighbour group test.
+  void force_prb(int nlocal, const double* x, const double* v, const int* type, const int* mask,
+                 const int* off, const int* nbr) {
+    if (eph_flag & FRICTION)
+      for (int i = 0; i < nlocal; ++i) {
+        if (!(mask[i] & groupbit)) continue;
+        int itype = type[i];
+        if (!(rho_i[i] > 0)) continue;
+        f_EPH[3 * i + 0] = v[3 * i + 0];
+        f_EPH[3 * i + 1] = v[3 * i + 1];
+        f_EPH[3 * i + 2] = v[3 * i + 2];
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+          int jj = nbr[k];
+          int jtype = type[jj];
+          double e[3];
+          double r_sq = dist_sq(&x[3 * jj], &x[3 * i], e);
+          if (r_sq < r_cutoff_sq) {
+            double var = beta.rho[jtype - 1](std::sqrt(r_sq)) / rho_i[i];
+            f_EPH[3 * i + 0] -= var * v[3 * jj + 0];
+            f_EPH[3 * i + 1] -= var * v[3 * jj + 1];
+            f_EPH[3 * i + 2] -= var * v[3 * jj + 2];
+          }
+        }
+        double var = beta.get_beta(type_map[itype - 1], rho_i[i]);
+        f_EPH[3 * i + 0] *= var;
+        f_EPH[3 * i + 1] *= var;
+        f_EPH[3 * i + 2] *= var;
+      }
+    random_per_atom(nlocal, x, type, mask);
   }
 
   // fix_eph.cpp:841-907.  xi_in: injected noise for local atoms ([nlocal][3],
@@ -774,12 +890,21 @@ struct OFix {
             ora_xi((uint32_t)seed, (uint32_t)tag[i], (uint64_t)step, &xi_i[3 * i]);
           }
         }
+      if (style == STYLE_COLOURED_EXP_V1) {  // fix_eph_coloured_exp_v1.cpp:645-652
+        for (int i = 0; i < nlocal; ++i) filter(&zi[3 * i], &xi_i[3 * i]);
+        forward(zi, 3, nlocal, nghost, owner);
+      }
       forward(xi_i, 3, nlocal, nghost, owner);
     }
     calculate_environment(nlocal, x, type, mask, off, nbr);
     forward(rho_i, 1, nlocal, nghost, owner);
-    if (eph_model == 4) force_prl(nlocal, nghost, x, v, type, mask, off, nbr, owner);
-    else if (eph_model != -1) { err = "only model 4 (PRL) is restated"; return false; }
+    switch (eph_model) {  // fix_eph.cpp:877-889
+      case 1: force_ttm(nlocal, x, v, type, mask); break;
+      case 2: force_prb(nlocal, x, v, type, mask, off, nbr); break;
+      case 4: force_prl(nlocal, nghost, x, v, type, mask, off, nbr, owner); break;
+      case -1: break;  // force_testing() is empty (:839)
+      default: err = "model not restated (3: the reference indexes rho by jtype - i)"; return false;
+    }
     if ((eph_flag & FRICTION) && !(eph_flag & NOFRICTION))
       for (int i = 0; i < nlocal; ++i)
         for (int c = 0; c < 3; ++c) f[3 * i + c] += f_EPH[3 * i + c];
@@ -1019,5 +1144,8 @@ int ora_set_threads(int n) {
 
 double ora_fix_alpha(void* h, int elem, double rho) { return static_cast<OFix*>(h)->beta.get_alpha(elem, rho); }
 double ora_fix_rho_r_sq(void* h, int elem, double r2) { return static_cast<OFix*>(h)->beta.get_rho_r_sq(elem, r2); }
+double ora_fix_rho_r(void* h, int elem, double r) { return static_cast<OFix*>(h)->beta.rho[elem](r); }
+double ora_fix_beta(void* h, int elem, double rho) { return static_cast<OFix*>(h)->beta.get_beta(elem, rho); }
+double ora_fix_zeta(void* h) { return static_cast<OFix*>(h)->zeta; }
 
 }  // extern "C"

@@ -88,6 +88,9 @@ def lib():
             "ora_fix_alpha": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
             "ora_set_threads": (C.c_int, [C.c_int]),
             "ora_fix_rho_r_sq": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_fix_rho_r": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_fix_beta": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_fix_zeta": (C.c_double, [C.c_void_p]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -252,7 +255,8 @@ def sincos2pi(u):
 
 
 class Fix:
-    """FixEPH restatement (fix_eph.cpp), model 4, one rank."""
+    """FixEPH restatement (fix_eph.cpp: models 1, 2, 4 and TESTING) and the
+    eph/coloured, eph/coloured/exp and eph/coloured/exp/v1 styles; one rank."""
 
     def __init__(self, args, ntypes, boxlo, boxhi, dt, boltz, ftm2v, masses=None):
         arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
@@ -307,6 +311,9 @@ class Fix:
     def post_run(self): return lib().ora_fix_post_run(self.h)
     def alpha(self, elem, rho): return lib().ora_fix_alpha(self.h, elem, rho)
     def rho_r_sq(self, elem, r2): return lib().ora_fix_rho_r_sq(self.h, elem, r2)
+    def rho_r(self, elem, r): return lib().ora_fix_rho_r(self.h, elem, r)
+    def beta(self, elem, rho): return lib().ora_fix_beta(self.h, elem, rho)
+    def zeta(self): return lib().ora_fix_zeta(self.h)
 
     def __del__(self):
         if getattr(self, "h", None):

@@ -54,7 +54,7 @@ class Pair:
 
     def __init__(self, oracle, ea, tmpdir, cells=4, species=1, T=300.0, disp=0.0, pka_eV=0.0,
                  dt=1e-3, grid=4, flags="7", seed=12345, hot_spot=None, md_seed=7,
-                 inject=False, C_e=3e-6, kappa=0.1):
+                 inject=False, C_e=3e-6, kappa=0.1, style="eph", model=4, tau0=None):
         self.ea = ea
         self.oracle = oracle
         a = 3.615
@@ -64,7 +64,8 @@ class Pair:
                                            hot_spot)
         elements = names if species > 1 else names * 1
         args = inputs.fix_args(beta, elements, seed=seed, flags=flags, grid=(grid, grid, grid),
-                               T_infile=T_in, C_e=C_e, kappa=kappa)
+                               T_infile=T_in, C_e=C_e, kappa=kappa, style=style, model=model,
+                               tau0=tau0)
         inf = self.md.info()
         lo, hi = inf["boxlo"], inf["boxhi"]
         masses = [inputs.MASS_CU] * species

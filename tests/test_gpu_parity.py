@@ -237,3 +237,31 @@ def test_grid_many_atoms_per_cell_deposit(oracle):
         for k in range(n):
             o.insert_energy(x[k], y[k], z[k], E[k])
         assert np.array_equal(g.get("dT"), o.get("dT")), dims
+
+
+@pytest.mark.parametrize("model,species", [(1, 1), (1, 4), (2, 1), (2, 4)])
+def test_models_ttm_and_prb(oracle, tmp_path, model, species):
+    """fix eph models 1 (TTM) and 2 (PRB) against the oracle, with rebuilds and
+    the T_e grid in the loop"""
+    p = Pair(oracle, ea, str(tmp_path), cells=6, species=species, T=600.0, pka_eV=300.0, grid=6,
+             dt=2.5e-4, hot_spot=(2000.0, 5.0), model=model)
+    for _ in range(25):
+        p.step()
+    _fix_state_close(p)
+
+
+@pytest.mark.parametrize("style", ["eph/coloured", "eph/coloured/exp", "eph/coloured/exp/v1"])
+def test_coloured_styles(oracle, tmp_path, style):
+    """the coloured styles' filter states persist over steps and rebuilds and
+    give the oracle's bits"""
+    p = Pair(oracle, ea, str(tmp_path), cells=6, species=2, T=600.0, pka_eV=300.0, grid=6,
+             dt=2.5e-4, hot_spot=(2000.0, 5.0), style=style, tau0=1e-3)
+    for _ in range(25):
+        p.step()
+    _fix_state_close(p)
+    if style != "eph/coloured":
+        for which in (0, 1):
+            st = p.gpu.filter_state(which)
+            assert np.abs(st).max() > 0, which
+            p.gpu.set_filter_state(which, st * 0.5)
+            assert np.array_equal(p.gpu.filter_state(which), st * 0.5)

@@ -81,7 +81,9 @@ def _args(tmp_path, **kw):
     (lambda a: a[:16] + ["/nonexistent.beta", "Cu"], -2),     # missing .beta
     (lambda a: a[:17] + ["Xx"], -1),                          # element not in file
     (lambda a: a[:10] + ["0"] + a[11:], -1),                  # non-positive grid
-    (lambda a: a[:5] + ["2"] + a[6:], -8),                    # PRB model: not built
+    (lambda a: a[:5] + ["3"] + a[6:], -8),                    # PRLCM: undefined in the reference
+    (lambda a: a[:2] + ["eph/other"] + a[3:], -1),            # unknown style
+    (lambda a: a[:2] + ["eph/coloured/exp"] + a[3:5] + ["0"] + a[6:], -1),  # tau0 <= 0
     (lambda a: a[:5] + ["7"] + a[6:], -1),                    # invalid model
     (lambda a: a[:13] + ["/nonexistent.T"] + a[14:], -2),     # missing grid file
 ])

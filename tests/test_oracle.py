@@ -269,10 +269,12 @@ def test_gaussian_stream_statistics(oracle):
 
 
 # --------------------------------------------------------------------------- fix
-def make_fix(oracle, tmp_path, species=1, flags="7", grid=(3, 3, 3), box=None, dt=1e-3, T_e=300.0):
+def make_fix(oracle, tmp_path, species=1, flags="7", grid=(3, 3, 3), box=None, dt=1e-3, T_e=300.0,
+             style="eph", model=4, tau0=None):
     p = str(tmp_path / f"s{species}.beta")
     names = inputs.workload_beta(p, species)
-    args = inputs.fix_args(p, names, flags=flags, grid=grid, T_e=T_e)
+    args = inputs.fix_args(p, names, flags=flags, grid=grid, T_e=T_e, style=style, model=model,
+                           tau0=tau0)
     lo, hi = box
     return oracle.Fix(args, species, lo, hi, dt, inputs.BOLTZ, inputs.FTM2V,
                       masses=[inputs.MASS_CU] * species)
@@ -416,3 +418,139 @@ def test_deposit_sum_rule_and_integrator(oracle, tmp_path):
     dtf = 0.5 * dt * inputs.FTM2V / inputs.MASS_CU
     assert np.allclose(s.v[:27], v0 + dtf * s.f[:27], rtol=1e-15)
     assert np.allclose(s.x[:27], x0 + dt * (v0 + dtf * s.f[:27]), rtol=1e-15)
+
+
+# ------------------------------------------------ models 1, 2 and coloured styles
+def test_ttm_model_closed_form(oracle, tmp_path):
+    """model 1 (fix_eph.cpp:468-503): f_EPH = -beta(rho_i) v_i and
+    f_RNG = eta alpha(rho_i) sqrt(T_e) xi_i, atom by atom"""
+    L = 9.0
+    x, v, t = random_cluster(27, L, 21, species=4)
+    s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+    fix = make_fix(oracle, tmp_path, 4, flags="3", box=([0, 0, 0], [L, L, L]), grid=(2, 2, 2),
+                   model=1)
+    xi = np.random.default_rng(4).normal(size=(27, 3))
+    fix.post_force(s, xi=xi)
+    rho = fix.get("rho", 27)
+    b = np.array([fix.beta(t[i] - 1, rho[i]) for i in range(27)])
+    a = np.array([fix.alpha(t[i] - 1, rho[i]) for i in range(27)])
+    assert np.all(b > 0) and np.all(rho > 0)
+    fE = fix.get("f_EPH", 27)
+    assert np.allclose(fE, -b[:, None] * v, rtol=1e-15, atol=0)
+    assert (fE * v).sum() < 0  # dissipative
+    eta = math.sqrt(2 * inputs.BOLTZ / 1e-3)
+    assert np.allclose(fix.get("f_RNG", 27), (eta * a * math.sqrt(300.0))[:, None] * xi, rtol=1e-15)
+    assert np.all(fix.get("w", 27) == 0)  # no w sum in this model
+
+
+def test_prb_model_against_direct_sum(oracle, tmp_path):
+    """model 2 (fix_eph.cpp:505-568): f_EPH,i = beta(rho_i) (v_i - sum_j rho(r_ij) v_j / rho_i)
+    with the rho(r) spline; on a uniform-velocity lattice it reduces to
+    beta(rho_i) u (1 - S / rho_i), S the shell sum of rho(r)"""
+    L = 9.0
+    x, v, t = random_cluster(27, L, 23)
+    s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+    fix = make_fix(oracle, tmp_path, 1, flags="1", box=([0, 0, 0], [L, L, L]), grid=(2, 2, 2),
+                   model=2)
+    fix.post_force(s)
+    n = 27
+    rho = fix.get("rho", n)
+    ref = np.zeros((n, 3))
+    for i in range(n):
+        acc = s.v[i].copy()
+        for k in range(s.nbr_off[i], s.nbr_off[i + 1]):
+            j = s.nbr_idx[k]
+            d = s.x[j] - s.x[i]
+            r2 = d @ d
+            if r2 < 4.5 ** 2:
+                acc = acc - fix.rho_r(0, math.sqrt(r2)) / rho[i] * s.v[j]
+        ref[i] = fix.beta(0, rho[i]) * acc
+    assert np.allclose(fix.get("f_EPH", n), ref, rtol=0, atol=1e-13 * np.abs(ref).max())
+    # uniform velocity on a perfect lattice
+    a0 = 3.615
+    xl = minisys.fcc(3, a0)
+    u = np.array([1.5, -0.5, 2.0])
+    sl = minisys.System(xl, np.tile(u, (len(xl), 1)), np.ones(len(xl), int), [0, 0, 0],
+                        [3 * a0] * 3, 6.5, None)
+    fl = make_fix(oracle, tmp_path, 1, flags="1", box=([0, 0, 0], [3 * a0] * 3), grid=(3, 3, 3),
+                  model=2)
+    fl.post_force(sl)
+    r0 = fl.get("rho", sl.nlocal)
+    S = sum(nn * fl.rho_r(0, d) for nn, d in [(12, a0 / math.sqrt(2)), (6, a0), (24, a0 * math.sqrt(1.5))])
+    closed = fl.beta(0, r0[0]) * u * (1.0 - S / r0[0])
+    assert np.allclose(fl.get("f_EPH", sl.nlocal), closed, rtol=1e-10, atol=1e-15)
+
+
+def _cluster_sequence(n_steps, seed):
+    rng = np.random.default_rng(seed)
+    L = 9.0
+    x, v, t = random_cluster(27, L, seed)
+    vs = [v + rng.normal(0, 1.0, v.shape) * k for k in range(n_steps)]
+    xis = [rng.normal(size=(27, 3)) for _ in range(n_steps)]
+    return L, x, t, vs, xis
+
+
+@pytest.mark.parametrize("style", ["eph/coloured", "eph/coloured/exp", "eph/coloured/exp/v1"])
+def test_coloured_styles_reduce_to_plain_prl(oracle, tmp_path, style):
+    """zeta = 1 - exp(-dt / tau0) = 1 for tau0 << dt: both filters pass the
+    new value through unchanged; eph/coloured runs PRL whatever the model slot
+    says (fix_eph_coloured.cpp:652)"""
+    L, x, t, vs, xis = _cluster_sequence(3, 31)
+    box = ([0, 0, 0], [L, L, L])
+    plain = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2))
+    col = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2), style=style, model=2,
+                   tau0=1e-300)
+    if style != "eph/coloured":
+        assert col.zeta() == 1.0
+    for v, xi in zip(vs, xis):
+        s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+        plain.post_force(s, xi=xi)
+        col.post_force(s, xi=xi)
+        for q in ("f_EPH", "f_RNG", "w"):
+            assert np.array_equal(plain.get(q, 27), col.get(q, 27)), q
+
+
+def test_coloured_exp_filter_recurrence(oracle, tmp_path):
+    """eph/coloured/exp (fix_eph_coloured_exp.cpp:562-568, 619-625): the applied
+    forces are f_dis <- f_dis (1 - zeta) + zeta f_EPH and the same for f_RNG,
+    with f_EPH, f_RNG the plain PRL forces of the step"""
+    L, x, t, vs, xis = _cluster_sequence(4, 33)
+    box = ([0, 0, 0], [L, L, L])
+    dt, tau0 = 1e-3, 3e-3
+    plain = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2), dt=dt)
+    col = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2), dt=dt,
+                   style="eph/coloured/exp", tau0=tau0)
+    zeta = 1.0 - math.exp(-dt / tau0)
+    assert col.zeta() == zeta
+    fd = np.zeros((27, 3)); fs = np.zeros((27, 3))
+    for v, xi in zip(vs, xis):
+        s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+        plain.post_force(s, xi=xi)
+        col.post_force(s, xi=xi)
+        fd = fd * (1. - zeta) + zeta * plain.get("f_EPH", 27)
+        fs = fs * (1. - zeta) + zeta * plain.get("f_RNG", 27)
+        assert np.array_equal(col.get("f_EPH", 27), fd)
+        assert np.array_equal(col.get("f_RNG", 27), fs)
+
+
+def test_coloured_exp_v1_filters_velocities_and_noise(oracle, tmp_path):
+    """eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp:477-487, 645-652): the
+    friction sums see zv <- zv (1 - zeta) + zeta v and the random sums
+    zi <- zi (1 - zeta) + zeta xi, i.e. plain PRL evaluated at (zv, zi)"""
+    L, x, t, vs, xis = _cluster_sequence(4, 35)
+    box = ([0, 0, 0], [L, L, L])
+    dt, tau0 = 1e-3, 2.5e-3
+    plain = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2), dt=dt)
+    col = make_fix(oracle, tmp_path, 1, flags="3", box=box, grid=(2, 2, 2), dt=dt,
+                   style="eph/coloured/exp/v1", tau0=tau0)
+    zeta = 1.0 - math.exp(-dt / tau0)
+    zv = np.zeros((27, 3)); zi = np.zeros((27, 3))
+    for v, xi in zip(vs, xis):
+        s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+        col.post_force(s, xi=xi)
+        zv = zv * (1. - zeta) + zeta * v
+        zi = zi * (1. - zeta) + zeta * xi
+        sz = minisys.System(x, zv, t, [0, 0, 0], [L, L, L], 6.5, None)
+        plain.post_force(sz, xi=zi)
+        assert np.array_equal(col.get("f_EPH", 27), plain.get("f_EPH", 27))
+        assert np.array_equal(col.get("f_RNG", 27), plain.get("f_RNG", 27))

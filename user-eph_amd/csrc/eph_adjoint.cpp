@@ -343,6 +343,10 @@ int eph_adjoint_create(eph_adjoint** out, eph_md* md, eph_context* ctx, int nste
     g_adj_error = "temperature-dependent C_e / kappa grids are not differentiated";
     return EPH_ERR_UNSUPPORTED;
   }
+  if (ctx->model != 4 || ctx->colour >= 2) {
+    g_adj_error = "only the PRL model of the eph and eph/coloured styles is differentiated";
+    return EPH_ERR_UNSUPPORTED;
+  }
   auto* a = new eph_adjoint();
   *out = a;
   a->md = md; a->ctx = ctx; a->nsteps = nsteps; a->every = every;

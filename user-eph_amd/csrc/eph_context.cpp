@@ -209,6 +209,7 @@ StepParams eph_context::step_params() {
   p.flags = flags; p.eta = eta; p.dtv = dtv; p.dtf = dtf; p.dt = dt;
   p.mass = mass.p; p.seed = static_cast<uint32_t>(seed); p.step = static_cast<uint64_t>(step);
   p.xi_mode = 0; p.xi_in = xi_in.p;
+  p.model = model; p.colour = colour; p.zeta = zeta;
   return p;
 }
 
@@ -217,7 +218,24 @@ WorkView eph_context::work_view() {
   w.nbr = nbr.p; w.nnbr = nnbr.p; w.act = act.p; w.nact = nact.p;
   w.rec = rec.p; w.pv = pv.p; w.fE = fE.p; w.fR = fR.p; w.flags_out = flags_dev.p;
   w.ghost_owner = ghost_owner.p; w.ghost_shift = ghost_shift.p;
+  w.cA = colA.p; w.cB = colB.p;
+  w.vw = colour == 3 ? colA.p : vel.p;
   return w;
+}
+
+// Filter states persist from step to step, so growing them keeps the old
+// entries (LAMMPS's grow_arrays) and zeroes the new ones.
+int eph_context::grow_filter_state(size_t nall) {
+  if (colour < 2 || (colA.p && colA.n >= nall)) return EPH_OK;
+  const size_t want = nall + nall / 8 + 64;
+  EPH_HIP(this, hipStreamSynchronize(stream));  // the old states are final
+  for (eph::DevArray<double4>* a : {&colA, &colB}) {
+    eph::DevArray<double4> b;
+    EPH_HIP(this, b.reserve(want));  // zero-filled
+    if (a->p && a->n) EPH_HIP(this, hipMemcpy(b.p, a->p, a->n * sizeof(double4), hipMemcpyDeviceToDevice));
+    *a = std::move(b);
+  }
+  return EPH_OK;
 }
 
 int eph_context::reserve_atoms(int nl, int ng) {
@@ -246,7 +264,8 @@ int eph_context::reserve_atoms(int nl, int ng) {
   fstride = st;
   nlocal = nl;
   nghost = ng;
-  return EPH_OK;
+  // the filter states cover every atom slot a later device rebuild may fill
+  return grow_filter_state(nall);
 }
 
 int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, const int* ty,
@@ -397,19 +416,31 @@ int eph_context::dev_post_force() {
   const AtomsView a = atoms_view();
   const WorkView w = work_view();
   hipEvent_t e;
+  if (colour == 3 && (flags & EPH_FRICTION)) EPH_HIP(this, launch_colour_v(a, p, w, stream));
   timed_begin("pack_pv", &e);
   EPH_HIP(this, launch_pack_pv(a, w, stream));
   timed_end("pack_pv", e);
+  // models other than PRL need rho (and xi) but no w sum
+  StepParams pd = p;
+  if (model != 4) pd.flags &= ~EPH_FRICTION;
   timed_begin("density_w", &e);
-  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
+  EPH_HIP(this, launch_density_w(a, tabs, pd, w, stream, form_density));
   timed_end("density_w", e);
-  timed_begin("ghost_rec", &e);
-  EPH_HIP(this, launch_ghost_rec(a, w, stream));
-  timed_end("ghost_rec", e);
   if (model == 4) {
+    timed_begin("ghost_rec", &e);
+    EPH_HIP(this, launch_ghost_rec(a, w, stream));
+    timed_end("ghost_rec", e);
     timed_begin("prl_forces", &e);
     EPH_HIP(this, launch_prl_forces(a, tabs, grid->v, p, w, stream, form_force));
     timed_end("prl_forces", e);
+  } else if (model == 1) {
+    timed_begin("ttm_forces", &e);
+    EPH_HIP(this, launch_ttm_forces(a, tabs, grid->v, p, w, stream));
+    timed_end("ttm_forces", e);
+  } else if (model == 2) {
+    timed_begin("prb_forces", &e);
+    EPH_HIP(this, launch_prb_forces(a, tabs, grid->v, p, w, stream));
+    timed_end("prb_forces", e);
   }
   xi_mode_next = 0;
   have_forces = true;
@@ -548,12 +579,29 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   c->device = device;
   // arguments and input files are checked on the host first (no device needed)
   // fix_eph.cpp:85-132
+  // fix styles with FixEPH's command layout: eph (fix_eph.cpp), eph/coloured
+  // (fix_eph_coloured.cpp, PRL whatever arg[5] says, :652), eph/coloured/exp
+  // and eph/coloured/exp/v1 (arg[5] = tau0, fix_eph_coloured_exp.cpp:113)
+  const std::string style = arg[2] ? arg[2] : "";
+  if (style == "eph" || style == "eph/gpu") c->colour = 0;
+  else if (style == "eph/coloured") c->colour = 1;
+  else if (style == "eph/coloured/exp") c->colour = 2;
+  else if (style == "eph/coloured/exp/v1") c->colour = 3;
+  else return bail(EPH_ERR_ARGS, "unknown fix style " + style);
   c->seed = std::atoi(arg[3]);
   c->flags = (int)std::strtol(arg[4], nullptr, 0);
-  c->model = std::atoi(arg[5]);
-  if (c->model == 1 || c->model == 2 || c->model == 3)
-    return bail(EPH_ERR_UNSUPPORTED, "only the PRL model (4) and TESTING (-1) are implemented");
-  if (c->model != 4 && c->model != -1) return bail(EPH_ERR_ARGS, "invalid eph model");
+  if (c->colour >= 2) {
+    c->tau0 = std::atof(arg[5]);
+    if (!(c->tau0 > 0)) return bail(EPH_ERR_ARGS, "tau0 must be positive");
+    c->model = 4;
+  } else {
+    c->model = c->colour == 1 ? 4 : std::atoi(arg[5]);
+  }
+  if (c->model == 3)
+    return bail(EPH_ERR_UNSUPPORTED,
+                "model 3 (PRLCM) indexes rho by jtype - i in the reference (fix_eph.cpp:600): undefined");
+  if (c->model != 1 && c->model != 2 && c->model != 4 && c->model != -1)
+    return bail(EPH_ERR_ARGS, "invalid eph model");
   const double v_rho = std::atof(arg[6]), v_Ce = std::atof(arg[7]);
   const double v_kappa = std::atof(arg[8]), v_Te = std::atof(arg[9]);
   const int nx = std::atoi(arg[10]), ny = std::atoi(arg[11]), nz = std::atoi(arg[12]);
@@ -600,22 +648,29 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   c->eta = std::sqrt(2.0 * boltz / dt);
   c->dtv = dt;
   c->dtf = 0.5 * dt * ftm2v;
+  if (c->colour >= 2) c->zeta = 1.0 - std::exp(-dt / c->tau0);  // fix_eph_coloured_exp.cpp:190
   // tables
   const int ne = (int)c->beta.names.size();
   const int nr = c->beta.n_rho, nb = c->beta.n_beta;
-  std::vector<Cubic> r2((size_t)ne * nr), al((size_t)ne * nb), be((size_t)ne * nb);
+  if (c->model == 2 && ntypes > ne)
+    return bail(EPH_ERR_ARGS, "model 2 evaluates rho(r) of element jtype - 1: the .beta file needs "
+                              "at least ntypes elements (fix_eph.cpp:537)");
+  std::vector<Cubic> r2((size_t)ne * nr), al((size_t)ne * nb), be((size_t)ne * nb), r1((size_t)ne * nr);
   for (int e = 0; e < ne; ++e) {
+    std::copy(c->beta.rho[e].knots.begin(), c->beta.rho[e].knots.end(), r1.begin() + (size_t)e * nr);
     std::copy(c->beta.rho_r_sq[e].knots.begin(), c->beta.rho_r_sq[e].knots.end(), r2.begin() + (size_t)e * nr);
     std::copy(c->beta.alpha[e].knots.begin(), c->beta.alpha[e].knots.end(), al.begin() + (size_t)e * nb);
     std::copy(c->beta.beta[e].knots.begin(), c->beta.beta[e].knots.end(), be.begin() + (size_t)e * nb);
   }
   if (c->t_rho_r2.reserve(r2.size()) || c->t_alpha.reserve(al.size()) || c->t_beta.reserve(be.size()) ||
+      c->t_rho_r.reserve(r1.size()) ||
       c->scal.reserve(4) || c->flags_dev.reserve(1) || c->mass.reserve(ntypes + 1))
     return bail(EPH_ERR_OOM, "device allocation failed");
   c->mass_host.assign(ntypes + 1, 1.0);
   if (hipMemcpyAsync(c->t_rho_r2.p, r2.data(), r2.size() * sizeof(Cubic), hipMemcpyHostToDevice, c->stream) ||
       hipMemcpyAsync(c->t_alpha.p, al.data(), al.size() * sizeof(Cubic), hipMemcpyHostToDevice, c->stream) ||
       hipMemcpyAsync(c->t_beta.p, be.data(), be.size() * sizeof(Cubic), hipMemcpyHostToDevice, c->stream) ||
+      hipMemcpyAsync(c->t_rho_r.p, r1.data(), r1.size() * sizeof(Cubic), hipMemcpyHostToDevice, c->stream) ||
       hipMemcpyAsync(c->mass.p, c->mass_host.data(), (ntypes + 1) * sizeof(double), hipMemcpyHostToDevice,
                      c->stream) ||
       hipStreamSynchronize(c->stream) != hipSuccess)
@@ -624,6 +679,7 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   t.n_elem = ne; t.n_rho = nr; t.inv_dr2 = c->beta.rho_r_sq[0].inv_dx; t.rho_r2 = c->t_rho_r2.p;
   t.n_beta = nb; t.inv_drho = c->beta.alpha[0].inv_dx; t.alpha = c->t_alpha.p; t.beta = c->t_beta.p;
   t.rc2 = c->beta.r_cutoff_sq; t.rho_cut = c->beta.rho_cutoff;
+  t.rho_r = c->t_rho_r.p; t.inv_dr = c->beta.rho[0].inv_dx;
   *out = c;
   return EPH_OK;
 }
@@ -661,7 +717,35 @@ int eph_reset_dt(eph_context* ctx, double dt) {
   ctx->eta = std::sqrt(2.0 * ctx->boltz / dt);
   ctx->dtv = dt;
   ctx->dtf = 0.5 * dt * ctx->ftm2v;
+  if (ctx->colour >= 2) ctx->zeta = 1.0 - std::exp(-dt / ctx->tau0);  // fix_eph_coloured_exp.cpp:685
   ctx->grid->h.dt = dt;
+  return EPH_OK;
+}
+
+int eph_get_filter_state(eph_context* ctx, int which, double* out) {
+  if (!ctx || !out || which < 0 || which > 1) return EPH_ERR_ARGS;
+  if (ctx->colour < 2) return ctx->fail(EPH_ERR_STATE, "the fix style keeps no filter state");
+  const int n = ctx->nlocal;
+  if (n == 0) return EPH_OK;
+  std::vector<double4> h(n);
+  const eph::DevArray<double4>& src = which == 0 ? ctx->colA : ctx->colB;
+  EPH_HIP(ctx, hipMemcpy(h.data(), src.p, n * sizeof(double4), hipMemcpyDeviceToHost));
+  for (int i = 0; i < n; ++i) { out[3 * i] = h[i].x; out[3 * i + 1] = h[i].y; out[3 * i + 2] = h[i].z; }
+  return EPH_OK;
+}
+
+int eph_set_filter_state(eph_context* ctx, int which, const double* in) {
+  if (!ctx || !in || which < 0 || which > 1) return EPH_ERR_ARGS;
+  if (ctx->colour < 2) return ctx->fail(EPH_ERR_STATE, "the fix style keeps no filter state");
+  const int n = ctx->nlocal;
+  if (n == 0) return EPH_OK;
+  int rc = ctx->grow_filter_state((size_t)n + ctx->nghost);
+  if (rc != EPH_OK) return rc;
+  std::vector<double4> h(n);
+  for (int i = 0; i < n; ++i) h[i] = make_double4(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.0);
+  eph::DevArray<double4>& dst = which == 0 ? ctx->colA : ctx->colB;
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  EPH_HIP(ctx, hipMemcpy(dst.p, h.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   return EPH_OK;
 }
 

@@ -111,18 +111,24 @@ struct eph_context {
   std::string err;
   // fix parameters (fix_eph.cpp:61-241)
   int seed = 0, flags = 0, model = 0, ntypes = 0, T_freq = 0, groupbit = 1;
+  // fix style: 0 eph, 1 eph/coloured, 2 eph/coloured/exp, 3 eph/coloured/exp/v1
+  int colour = 0;
+  double tau0 = 0, zeta = 0;
   std::string T_out, T_state;
   double dt = 0, boltz = 0, ftm2v = 0, eta = 0, dtv = 0, dtf = 0;
   std::vector<int> type_map;
   std::vector<double> mass_host;
   eph::BetaModel beta;
   eph::TablesView tabs;
-  eph::DevArray<double4> t_rho_r2, t_alpha, t_beta;
+  eph::DevArray<double4> t_rho_r2, t_alpha, t_beta, t_rho_r;
   eph_grid* grid = nullptr;
   // atoms
   int nlocal = 0, nghost = 0, stride = 0, fstride = 0, maxnbr = 0;
   bool have_atoms = false, have_list = false, have_forces = false;
   eph::DevArray<double4> pos, vel, ghost_shift, stage4;
+  // filter states of the coloured styles, kept across steps (and across
+  // set_atoms calls: entries beyond the old size start at zero)
+  eph::DevArray<double4> colA, colB;
   eph::DevArray<eph::ForceRec> rec;
   eph::DevArray<eph::PosVel> pv;
   // field f of every ForceRec (0 position, 1 (w, rho), 2 (xi, alpha)), stride 4 double4
@@ -163,6 +169,7 @@ struct eph_context {
   int dev_end_of_step(long long ntimestep);
   int dev_deposit();
   int autotune(int* forms, double* ms);
+  int grow_filter_state(size_t nall);
   int check_flags();
   // timing helpers
   hipEvent_t take_event();

@@ -31,6 +31,8 @@ struct TablesView {
   const double4* beta = nullptr;    // [n_elem][n_beta]
   double rc2 = 0;
   double rho_cut = 0;
+  const double4* rho_r = nullptr;   // rho(r) tables [n_elem][n_rho] (model 2)
+  double inv_dr = 0;                // 1 / dr
 };
 
 struct GridView {
@@ -76,6 +78,9 @@ struct StepParams {
   uint64_t step = 0;
   int xi_mode = 0;       // 0: zero, 1: injected [nlocal][3], 2: Philox stream
   const double* xi_in = nullptr;
+  int model = 4;         // fix eph model: 1 TTM, 2 PRB, 4 PRL, -1 TESTING
+  int colour = 0;        // fix style: 0 eph, 1 eph/coloured, 2 .../exp, 3 .../exp/v1
+  double zeta = 0;       // exponential filter weight 1 - exp(-dt / tau0) (styles 2, 3)
 };
 
 // Per-atom record of the force sweep: position (with element), (w, rho) and
@@ -101,6 +106,11 @@ struct WorkView {
   int* flags_out = nullptr;   // bit 0: rho > rho_cutoff seen
   const int* ghost_owner = nullptr;
   const double4* ghost_shift = nullptr;
+  // filter states of the coloured styles: (f_dis, f_sto) for eph/coloured/exp,
+  // (zv, zi) for eph/coloured/exp/v1; .w unused
+  double4* cA = nullptr;
+  double4* cB = nullptr;
+  const double4* vw = nullptr;  // velocities the w sweep uses (vel, or zv)
 };
 
 // Ordered deposit of per-atom values into grid cells (bit-identical to a
@@ -148,6 +158,13 @@ hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
                              const StepParams& p, const WorkView& w, hipStream_t s, int form = -1);
 hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s);
+// models 1 (TTM) and 2 (PRB) of fix eph, after the density sweep
+hipError_t launch_ttm_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s);
+hipError_t launch_prb_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s);
+// eph/coloured/exp/v1: zv <- zv (1 - zeta) + zeta v for locals, then ghosts <- owners
+hipError_t launch_colour_v(const AtomsView& a, const StepParams& p, const WorkView& w, hipStream_t s);
 hipError_t launch_fill_int(int* p, int n, int v, hipStream_t s);
 hipError_t launch_fill_double(double* p, long n, double v, hipStream_t s);
 // deposit of the fix (energy of friction / random forces at end of step)

@@ -458,6 +458,12 @@ int eph_ensemble_add(eph_ensemble* e, const eph_md_config* cfg, int narg, const 
     eph_md_destroy(r.md);
     return rc;
   }
+  if (r.fix->colour >= 2 || r.fix->model != 4) {
+    e->err = "ensemble replicas run the PRL model of the eph or eph/coloured styles";
+    eph_destroy(r.fix);
+    eph_md_destroy(r.md);
+    return EPH_ERR_UNSUPPORTED;
+  }
   if ((rc = eph_set_masses(r.fix, masses, ntypes))) {
     e->err = r.fix->err;
     eph_destroy(r.fix);

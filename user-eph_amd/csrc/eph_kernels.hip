@@ -122,7 +122,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   }
   double wx = 0.0, wy = 0.0, wz = 0.0;
   if (FRIC && grp && rho > 0.0) {
-    const double4 vi = a.vel[i];
+    const double4 vi = w.vw[i];
     int jn = na > 0 ? w.act[i] : 0;
     double4 pn = pi, vn = vi;
     if (na > 0) {
@@ -160,6 +160,18 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
     double g3[3];
     gaussian3(p.seed, static_cast<uint32_t>(a.tag[i]), p.step, g3);
     x0 = g3[0]; x1 = g3[1]; x2 = g3[2];
+  }
+  if (XI != 0 && p.colour == 3) {
+    // eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp:645-652): every local
+    // atom's noise is filtered, zi <- zi (1 - zeta) + zeta xi, and the random
+    // sums use zi
+    const double om = 1. - p.zeta;
+    double4 z = w.cB[i];
+    z.x = z.x * om + p.zeta * x0;
+    z.y = z.y * om + p.zeta * x1;
+    z.z = z.z * om + p.zeta * x2;
+    w.cB[i] = z;
+    x0 = z.x; x1 = z.y; x2 = z.z;
   }
   ForceRec* r = w.rec + i;
   r->p = pi;
@@ -232,7 +244,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_pv(AtomsView a, WorkView w, int
   if (i >= n) return;
   PosVel q;
   q.p = a.pos[i];
-  q.v = a.vel[i];
+  q.v = w.vw[i];
   w.pv[i] = q;
 }
 
@@ -314,6 +326,27 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
       fry *= var;
       frz *= var;
     }
+    if (p.colour == 2) {
+      // eph/coloured/exp (fix_eph_coloured_exp.cpp:562-568, 619-625): the
+      // forces applied and deposited are the filtered f_dis, f_sto
+      const double om = 1. - p.zeta;
+      if (FRIC) {
+        double4 d = w.cA[i];
+        d.x = d.x * om + p.zeta * fex;
+        d.y = d.y * om + p.zeta * fey;
+        d.z = d.z * om + p.zeta * fez;
+        w.cA[i] = d;
+        fex = d.x; fey = d.y; fez = d.z;
+      }
+      if (RAND) {
+        double4 d = w.cB[i];
+        d.x = d.x * om + p.zeta * frx;
+        d.y = d.y * om + p.zeta * fry;
+        d.z = d.z * om + p.zeta * frz;
+        w.cB[i] = d;
+        frx = d.x; fry = d.y; frz = d.z;
+      }
+    }
   }
   const int fs = a.fstride;
   if (FRIC) {
@@ -354,6 +387,132 @@ __global__ __launch_bounds__(kBigBlock) void k_prl_forces_lds(AtomsView a, Table
     const int i = tile * kBigBlock + threadIdx.x;
     if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, s_tab, g, p, w);
   }
+}
+
+// ---------------------------------------------- fix eph models 1 and 2
+// beta(rho) and alpha(rho) with the reference's cutoff rule: zero (and the
+// warning flag) above rho_cutoff (eph_beta.h:171-198)
+__device__ __forceinline__ double beta_at(const TablesView& t, int e, double rho, int* flag) {
+  if (rho > t.rho_cut) {
+    atomicOr(flag, 1);
+    return 0.;
+  }
+  return cubic_at(t.beta + e * t.n_beta, t.n_beta, t.inv_drho, rho);
+}
+__device__ __forceinline__ double alpha_at(const TablesView& t, int e, double rho, int* flag) {
+  if (rho > t.rho_cut) {
+    atomicOr(flag, 1);
+    return 0.;
+  }
+  return cubic_at(t.alpha + e * t.n_beta, t.n_beta, t.inv_drho, rho);
+}
+
+// the random force of models 1 and 2 (fix_eph.cpp:489-502, 554-567):
+// f_RNG = eta alpha(rho_i) sqrt(T_e) xi_i
+__device__ __forceinline__ void random_per_atom(const int i, const TablesView& t, const GridView& g,
+                                                const StepParams& p, const WorkView& w,
+                                                const double4& pi, double rho, double* fr) {
+  const int ei = static_cast<int>(pi.w);
+  const double Te = g.T[cell_of(g, pi.x, pi.y, pi.z)];
+  const double var = p.eta * alpha_at(t, ei, rho, w.flags_out) * sqrt(Te);
+  const double4 xa = w.rec[i].xa;
+  fr[0] = var * xa.x;
+  fr[1] = var * xa.y;
+  fr[2] = var * xa.z;
+}
+
+// f_EPH, f_RNG out; f += them unless NOFRICTION / NORANDOM (fix_eph.cpp:892-906)
+__device__ __forceinline__ void store_forces(const int i, const AtomsView& a, const StepParams& p,
+                                             const WorkView& w, bool fric, bool rnd,
+                                             const double* fe, const double* fr) {
+  const int fs = a.fstride;
+  if (fric) { w.fE[i] = fe[0]; w.fE[fs + i] = fe[1]; w.fE[2 * fs + i] = fe[2]; }
+  if (rnd) { w.fR[i] = fr[0]; w.fR[fs + i] = fr[1]; w.fR[2 * fs + i] = fr[2]; }
+  const bool addE = fric && !(p.flags & 0x10);
+  const bool addR = rnd && !(p.flags & 0x20);
+  if (addE || addR) {
+    double fx = a.f[i], fy = a.f[fs + i], fz = a.f[2 * fs + i];
+    if (addE) { fx = fx + fe[0]; fy = fy + fe[1]; fz = fz + fe[2]; }
+    if (addR) { fx = fx + fr[0]; fy = fy + fr[1]; fz = fz + fr[2]; }
+    a.f[i] = fx; a.f[fs + i] = fy; a.f[2 * fs + i] = fz;
+  }
+}
+
+// model 1, FixEPH::force_ttm (fix_eph.cpp:468-503): f_EPH = -beta(rho_i) v_i
+template <bool FRIC, bool RAND>
+__global__ __launch_bounds__(kBlock) void k_ttm_forces(AtomsView a, TablesView t, GridView g,
+                                                       StepParams p, WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  double fe[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
+  if ((a.mask[i] & a.groupbit) != 0) {
+    const double4 pi = a.pos[i];
+    const double rho = w.rec[i].rw.w;
+    if (FRIC) {
+      const double var = -beta_at(t, static_cast<int>(pi.w), rho, w.flags_out);
+      const double4 v = a.vel[i];
+      fe[0] = var * v.x; fe[1] = var * v.y; fe[2] = var * v.z;
+    }
+    if (RAND) random_per_atom(i, t, g, p, w, pi, rho, fr);
+  }
+  store_forces(i, a, p, w, FRIC, RAND, fe, fr);
+}
+
+// model 2, FixEPH::force_prb (fix_eph.cpp:505-568): f_EPH = beta(rho_i)
+// (v_i - sum_j rho_{jtype-1}(r_ij) v_j / rho_i) over the entries with
+// r^2 < r_c^2 in list order (the density sweep's active list), rho(r) of the
+// unmapped element jtype - 1 at r = sqrt(r^2)
+template <bool FRIC, bool RAND>
+__global__ __launch_bounds__(kBlock) void k_prb_forces(AtomsView a, TablesView t, GridView g,
+                                                       StepParams p, WorkView w) {
+  const int i = xcd_block() * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  double fe[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
+  if ((a.mask[i] & a.groupbit) != 0) {
+    const double4 pi = a.pos[i];
+    const double rho = w.rec[i].rw.w;
+    if (FRIC && rho > 0.0) {
+      const double4 vi = a.vel[i];
+      fe[0] = vi.x; fe[1] = vi.y; fe[2] = vi.z;
+      const int na = w.nact[i];
+      for (int k = 0; k < na; ++k) {
+        const int j = w.act[k * a.stride + i];
+        const double4 pj = a.pos[j];
+        const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        const double r2 = ex * ex + ey * ey + ez * ez;
+        const int ej = a.type[j] - 1;
+        const double var = cubic_at(t.rho_r + ej * t.n_rho, t.n_rho, t.inv_dr, sqrt(r2)) / rho;
+        const double4 vj = a.vel[j];
+        fe[0] -= var * vj.x;
+        fe[1] -= var * vj.y;
+        fe[2] -= var * vj.z;
+      }
+      const double b = beta_at(t, static_cast<int>(pi.w), rho, w.flags_out);
+      fe[0] *= b; fe[1] *= b; fe[2] *= b;
+    }
+    if (RAND) random_per_atom(i, t, g, p, w, pi, rho, fr);
+  }
+  store_forces(i, a, p, w, FRIC, RAND, fe, fr);
+}
+
+// eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp:477-487): every local
+// atom's velocity filtered, then the ghosts take their owners' values
+__global__ __launch_bounds__(kBlock) void k_colour_v_local(AtomsView a, StepParams p, WorkView w) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= a.nlocal) return;
+  const double om = 1. - p.zeta;
+  const double4 v = a.vel[i];
+  double4 z = w.cA[i];
+  z.x = z.x * om + p.zeta * v.x;
+  z.y = z.y * om + p.zeta * v.y;
+  z.z = z.z * om + p.zeta * v.z;
+  w.cA[i] = z;
+}
+
+__global__ __launch_bounds__(kBlock) void k_colour_v_ghost(AtomsView a, WorkView w) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= a.nghost) return;
+  w.cA[a.nlocal + g] = w.cA[w.ghost_owner[g]];
 }
 
 // --------------------------------------------------------- integrator
@@ -1075,7 +1234,7 @@ __global__ __launch_bounds__(kBlock) void k_pack_pv_dev(AtomsView a, WorkView w,
   if (i >= a.nlocal + *ng) return;
   PosVel q;
   q.p = a.pos[i];
-  q.v = a.vel[i];
+  q.v = w.vw[i];
   w.pv[i] = q;
 }
 
@@ -1254,6 +1413,34 @@ hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const Grid
   else { EPH_PF_M(false, true); }
 #undef EPH_PF_M
 #undef EPH_PF
+  return hipGetLastError();
+}
+
+hipError_t launch_ttm_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02;
+  const int gr = nblk(a.nlocal);
+  if (fric && rnd) k_ttm_forces<true, true><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  else if (fric) k_ttm_forces<true, false><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  else if (rnd) k_ttm_forces<false, true><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_prb_forces(const AtomsView& a, const TablesView& t, const GridView& g,
+                             const StepParams& p, const WorkView& w, hipStream_t s) {
+  if (a.nlocal == 0) return hipSuccess;
+  const bool fric = p.flags & 0x01, rnd = p.flags & 0x02;
+  const int gr = nblk(a.nlocal);
+  if (fric && rnd) k_prb_forces<true, true><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  else if (fric) k_prb_forces<true, false><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  else if (rnd) k_prb_forces<false, true><<<gr, kBlock, 0, s>>>(a, t, g, p, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_colour_v(const AtomsView& a, const StepParams& p, const WorkView& w, hipStream_t s) {
+  if (a.nlocal > 0) k_colour_v_local<<<nblk(a.nlocal), kBlock, 0, s>>>(a, p, w);
+  if (a.nghost > 0) k_colour_v_ghost<<<nblk(a.nghost), kBlock, 0, s>>>(a, w);
   return hipGetLastError();
 }
 

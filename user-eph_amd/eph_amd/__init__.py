@@ -78,6 +78,8 @@ _SIG = {
     "eph_stream": (vp, [vp]),
     "eph_timing_enable": (C.c_int, [vp, C.c_int]),
     "eph_set_sweep_forms": (C.c_int, [vp, C.c_int, C.c_int]),
+    "eph_get_filter_state": (C.c_int, [vp, C.c_int, dp]),
+    "eph_set_filter_state": (C.c_int, [vp, C.c_int, dp]),
     "eph_autotune": (C.c_int, [vp, ip, dp]),
     "eph_timing_report": (C.c_int, [vp, C.c_int, C.c_char_p, C.c_int, dp, C.POINTER(C.c_longlong)]),
     "eph_copy_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, dp]),
@@ -338,6 +340,16 @@ class FixEPH:
         return x, v, f
 
     def timing(self, on=True): self._check(lib().eph_timing_enable(self.h, 1 if on else 0))
+
+    def filter_state(self, which):
+        """coloured styles: 0 f_dis / zv, 1 f_sto / zi, [nlocal][3]"""
+        out = np.zeros((self.nlocal, 3))
+        self._check(lib().eph_get_filter_state(self.h, which, _d(out)))
+        return out
+
+    def set_filter_state(self, which, values):
+        a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.nlocal, 3)
+        self._check(lib().eph_set_filter_state(self.h, which, _d(a)))
 
     _FORMS = {"chunk": 0, "lds": 1, "auto": -1}
 

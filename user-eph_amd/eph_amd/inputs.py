@@ -100,10 +100,13 @@ def write_grid(path, nx, ny, nz, box, T, S=None, rho=None, C_e=None, kappa=None,
 
 def fix_args(beta_file, elements, seed=12345, flags="7", model=4, rho_e=1.0, C_e=3e-6,
              kappa=0.1, T_e=300.0, grid=(10, 10, 10), T_infile="NULL", T_freq=0,
-             T_out="Te_out"):
-    """`fix ID group eph seed flags model rho_e C_e kappa T_e nx ny nz T_infile freq T_out
-    beta_file elem...` (USER-EPH README / fix_eph.cpp:36-58)."""
-    return ["e", "all", "eph", str(seed), str(flags), str(model), repr(rho_e), repr(C_e),
+             T_out="Te_out", style="eph", tau0=None):
+    """`fix ID group <style> seed flags model|tau0 rho_e C_e kappa T_e nx ny nz T_infile freq
+    T_out beta_file elem...` (USER-EPH README / fix_eph.cpp:36-58).  The styles
+    eph/coloured/exp and eph/coloured/exp/v1 take tau0 (ps) in the model slot
+    (fix_eph_coloured_exp.cpp:43)."""
+    slot = repr(float(tau0)) if style.startswith("eph/coloured/exp") else str(model)
+    return ["e", "all", style, str(seed), str(flags), slot, repr(rho_e), repr(C_e),
             repr(kappa), repr(T_e), str(grid[0]), str(grid[1]), str(grid[2]), T_infile,
             str(T_freq), T_out, beta_file] + list(elements)
 
