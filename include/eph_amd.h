@@ -198,6 +198,16 @@ int eph_spline_build(double dx, const double* y, int n, double* coeffs);
 int eph_beta_table(const char* path, int elem, int which, double* coeffs, double* inv_dx,
                    double* r_cutoff, double* rho_cutoff);
 
+/* EPH_kappa(file) of eph_kappa.h:55-150 (fix eph/atomic): element count, the
+ * reference's pair-slot count (n + 1)(n - 1) / 2 (1 for one element), r_cutoff
+ * and T_max. */
+int eph_kappa_info(const char* path, int* n_elem, int* n_pairs, double* r_cutoff, double* T_max);
+/* table `which` of a .kappa file: 0 rho(r), 1 rho(r^2) of element `index`
+ * (out [n][4] coefficients, *dx = 1 / knot spacing); 2 E(T) of element
+ * `index`, 3 K(T) of pair slot `index` (out [n][2] = y, forward slope with 0
+ * at the top knot; *dx = knot spacing).  Returns n or a negative status. */
+int eph_kappa_table(const char* path, int index, int which, double* out, double* dx);
+
 /* --------------------------------------------------------------- T_e grid */
 /* EPH_FDM counterpart (eph_fdm.h); also usable standalone. */
 

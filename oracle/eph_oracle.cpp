@@ -957,6 +957,417 @@ struct OFix {
   }
 };
 
+
+// ---------------------------------------------------------------------------
+// EPH_kappa restatement: eph_kappa.h:55-150 (fix eph/atomic's .kappa file)
+// ---------------------------------------------------------------------------
+struct OKappa {
+  size_t n_elements = 0, n_pairs = 0;
+  double r_cutoff = 0, r_cutoff_sq = 0, T_max = 0;
+  std::vector<std::string> names;
+  std::vector<int> numbers;
+  std::vector<OSpline> rho_r, rho_r_sq;
+  std::vector<OLinear> E_T, K_T;
+
+  bool load(const char* path, std::string& err) {
+    std::ifstream fd(path);
+    if (!fd.is_open()) { err = std::string("cannot open ") + path; return false; }
+    std::string line;
+    for (int k = 0; k < 3; ++k) std::getline(fd, line);    // :63-65
+    fd >> n_elements;                                        // :68
+    if (!fd || n_elements < 1) { err = "bad element count"; return false; }
+    n_pairs = n_elements > 1 ? (n_elements + 1) * (n_elements - 1) / 2 : 1;  // :72
+    std::getline(fd, line);                                  // :82-91
+    {
+      std::istringstream ss(line);
+      for (size_t i = 0; i < n_elements; ++i) { std::string e; ss >> e; names.push_back(e); }
+    }
+    size_t n_r, n_T; double dr, dT;
+    fd >> n_r >> dr >> r_cutoff >> n_T >> dT >> T_max;      // :99-105
+    if (!fd) { err = "bad kappa header"; return false; }
+    r_cutoff_sq = r_cutoff * r_cutoff;                       // :107
+    double dr_sq = r_cutoff_sq / (static_cast<double>(n_r - 1));
+    std::vector<double> l_rho(n_r), l_C(n_T), l_K(n_T);
+    for (size_t i = 0; i < n_elements; ++i) {               // :114-139
+      int val; fd >> val; numbers.push_back(val);
+      for (size_t j = 0; j < n_r; ++j) fd >> l_rho[j];
+      rho_r.emplace_back(dr, l_rho);
+      for (size_t j = 0; j < n_r; ++j) l_rho[j] = rho_r[i](std::sqrt(j * dr_sq));
+      rho_r_sq.emplace_back(dr_sq, l_rho);
+      for (size_t j = 0; j < n_T; ++j) fd >> l_C[j];
+      l_C[0] = 0.;
+      for (size_t j = 1; j < n_T; ++j) l_C[j] = l_C[j - 1] + l_C[j] * dT;
+      E_T.emplace_back(dT, l_C);
+      if (!fd) { err = "truncated .kappa file"; return false; }
+    }
+    for (size_t k = 0; k < n_pairs; ++k) {                  // :142-148
+      for (size_t j = 0; j < n_T; ++j) fd >> l_K[j];
+      K_T.emplace_back(dT, l_K);
+      if (!fd) { err = "truncated .kappa file"; return false; }
+    }
+    return true;
+  }
+};
+
+// ---------------------------------------------------------------------------
+// FixEPHAtomic restatement: fix_eph_atomic.cpp:59-964 (one rank; forward
+// communication = ghost <- owner).  The reference's element checks reject an
+// element index larger than ntypes (:238); here only a missing element is
+// rejected, and a K(T) slot index beyond the reference's pair count (two-
+// element .kappa files) is rejected instead of read out of range.
+// ---------------------------------------------------------------------------
+struct OFixAtomic {
+  enum { A_FRICTION = 0x01, A_RANDOM = 0x02, A_HEAT = 0x04, A_NOINT = 0x08, A_NOFRICTION = 0x10,
+         A_NORANDOM = 0x20 };
+  std::string err;
+  int seed = 0, eph_flag = 0, types = 0, inner_loops = 0, groupbit = 1;
+  bool init_from_T = true, initialised = false;
+  double T0 = 0, dtv = 0, dtf = 0, eta_factor = 0, boltz = 0, ftm2v = 0;
+  double r_cutoff = 0, r_cutoff_sq = 0, rho_cutoff = 0;
+  double Ee = 0, Te = 0;
+  OBeta beta;
+  OKappa kappa;
+  std::vector<int> type_map_beta, type_map_kappa;
+  std::vector<double> mass;
+  std::vector<double> rho_i, rho_a_i, w_i, xi_i, f_EPH, f_RNG, E0, E1, dE, T_a, array;
+
+  // fix_eph_atomic.cpp:59-269
+  bool create(int narg, const char* const* arg, int ntypes, double dt, double boltz_, double ftm2v_) {
+    if (narg < 12) { err = "fix_eph_atomic: too few arguments"; return false; }
+    seed = std::atoi(arg[3]);
+    eph_flag = (int)std::strtol(arg[4], nullptr, 0);
+    inner_loops = std::atoi(arg[7]);
+    if (inner_loops < 1) inner_loops = 0;
+    types = ntypes;
+    if (types > narg - 11) { err = "fix_eph_atomic: number of types larger than provided in fix"; return false; }
+    if (!beta.load(arg[9], err)) return false;
+    if (!kappa.load(arg[10], err)) return false;
+    r_cutoff = beta.r_cutoff; r_cutoff_sq = beta.r_cutoff_sq; rho_cutoff = beta.rho_cutoff;
+    type_map_beta.assign(types, -1);
+    type_map_kappa.assign(types, -1);
+    for (int i = 0; i < types; ++i) {
+      for (size_t j = 0; j < beta.n_elements; ++j)
+        if (beta.names[j] == arg[11 + i]) { type_map_beta[i] = (int)j; break; }
+      for (size_t j = 0; j < kappa.n_elements; ++j)
+        if (kappa.names[j] == arg[11 + i]) { type_map_kappa[i] = (int)j; break; }
+      if (type_map_beta[i] < 0 || type_map_kappa[i] < 0) {
+        err = "fix_eph_atomic: elements not found in input file"; return false;
+      }
+      if ((size_t)type_map_kappa[i] >= kappa.n_pairs) {
+        err = "fix_eph_atomic: K(T) slot of an element beyond the .kappa pair count"; return false;
+      }
+    }
+    T0 = std::atof(arg[5]);
+    init_from_T = std::strcmp("NULL", arg[6]) == 0;
+    boltz = boltz_; ftm2v = ftm2v_;
+    reset_dt(dt);
+    mass.assign(types + 1, 1.0);
+    return true;
+  }
+  void reset_dt(double dt) {  // :826-831
+    eta_factor = std::sqrt(2.0 * boltz / dt);
+    dtv = dt;
+    dtf = 0.5 * dt * ftm2v;
+  }
+  void grow(size_t nall) {
+    if (rho_i.size() < nall) {
+      rho_i.resize(nall, 0.0); rho_a_i.resize(nall, 0.0); w_i.resize(3 * nall, 0.0);
+      xi_i.resize(3 * nall, 0.0); f_EPH.resize(3 * nall, 0.0); f_RNG.resize(3 * nall, 0.0);
+      E0.resize(nall, 0.0); E1.resize(nall, 0.0); dE.resize(nall, 0.0); T_a.resize(nall, 0.0);
+      array.resize(12 * nall, 0.0);
+    }
+  }
+  int kmap(int type) const { return type_map_kappa[type - 1]; }
+  double T_of(int type, double E) const { return kappa.E_T[kmap(type)].reverse_lookup(E); }
+  void forward(std::vector<double>& arr, int width, int nlocal, int nghost, const int* owner) {
+    for (int g = 0; g < nghost; ++g)
+      for (int c = 0; c < width; ++c) arr[(size_t)width * (nlocal + g) + c] = arr[(size_t)width * owner[g] + c];
+  }
+  static double dist_sq(const double* a, const double* b, double* e) {  // fix_eph_atomic.h:163-169
+    e[0] = a[0] - b[0]; e[1] = a[1] - b[1]; e[2] = a[2] - b[2];
+    return e[0] * e[0] + e[1] * e[1] + e[2] * e[2];
+  }
+  static double dot(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+  // the constructor's per-atom set-up with the first atoms (:242-268)
+  void initialise(int nlocal, const int* type, const int* mask) {
+    grow((size_t)nlocal);
+    if (init_from_T)
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit) E0[i] = kappa.E_T[kmap(type[i])](T0);
+    Ee = 0.;
+    for (int i = 0; i < nlocal; ++i)
+      if (mask[i] & groupbit) Ee += E0[i];
+    Te = 0.0;
+    int count = 0;
+    for (int i = 0; i < nlocal; ++i)
+      if (mask[i] & groupbit) { T_a[i] = T_of(type[i], E0[i]); Te += T_a[i]; count++; }
+    if (count > 0) Te /= static_cast<double>(count);
+    Te /= static_cast<double>(count > 0 ? 1 : 0);
+    populate_array(nlocal, type, mask);
+    initialised = true;
+  }
+
+  void initial_integrate(int nlocal, double* x, double* v, const double* f, const int* type,
+                         const int* mask) {  // :334-357
+    if (eph_flag & A_NOINT) return;
+    for (int i = 0; i < nlocal; ++i)
+      if (mask[i] & groupbit) {
+        double dtfm = dtf / mass[type[i]];
+        for (int c = 0; c < 3; ++c) v[3 * i + c] += dtfm * f[3 * i + c];
+        for (int c = 0; c < 3; ++c) x[3 * i + c] += dtv * v[3 * i + c];
+      }
+  }
+  void final_integrate(int nlocal, double* v, const double* f, const int* type, const int* mask) {  // :359-377
+    if (eph_flag & A_NOINT) return;
+    for (int i = 0; i < nlocal; ++i)
+      if (mask[i] & groupbit) {
+        double dtfm = dtf / mass[type[i]];
+        for (int c = 0; c < 3; ++c) v[3 * i + c] += dtfm * f[3 * i + c];
+      }
+  }
+
+  // :450-490
+  void calculate_environment(int nlocal, const double* x, const int* type, const int* mask,
+                             const int* off, const int* nbr) {
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < nlocal; ++i) {
+      rho_i[i] = 0;
+      rho_a_i[i] = 0;
+      if (!(mask[i] & groupbit)) continue;
+      for (int k = off[i]; k < off[i + 1]; ++k) {
+        int jj = nbr[k];
+        if (!(mask[jj] & groupbit)) continue;
+        int jtype = type[jj];
+        double e[3];
+        double r_sq = dist_sq(&x[3 * jj], &x[3 * i], e);
+        if (r_sq < r_cutoff_sq) rho_i[i] += beta.get_rho_r_sq(type_map_beta[jtype - 1], r_sq);
+        if (r_sq < kappa.r_cutoff_sq) rho_a_i[i] += kappa.rho_r_sq[kmap(jtype)](r_sq);
+      }
+    }
+  }
+
+  // :492-658
+  void force_prl(int nlocal, int nghost, const double* x, const double* v, const int* type,
+                 const int* mask, const int* off, const int* nbr, const int* owner) {
+    const double l_dt = dtv;
+    if (eph_flag & A_FRICTION) {
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < nlocal; ++i) {
+        if (!(mask[i] & groupbit)) continue;
+        int itype = type[i];
+        if (!(rho_i[i] > 0)) continue;
+        double alpha_i = beta.get_alpha(type_map_beta[itype - 1], rho_i[i]);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+          int jj = nbr[k];
+          int jtype = type[jj];
+          if (!(mask[jj] & groupbit)) continue;
+          double e_ij[3];
+          double e_r_sq = dist_sq(&x[3 * jj], &x[3 * i], e_ij);
+          if (e_r_sq >= r_cutoff_sq) continue;
+          double v_rho_ji = beta.get_rho_r_sq(type_map_beta[jtype - 1], e_r_sq);
+          double prescaler = alpha_i * v_rho_ji / (rho_i[i] * e_r_sq);
+          double var1 = prescaler * dot(e_ij, &v[3 * i]);
+          double var2 = prescaler * dot(e_ij, &v[3 * jj]);
+          double dvar = var1 - var2;
+          w_i[3 * i + 0] += dvar * e_ij[0];
+          w_i[3 * i + 1] += dvar * e_ij[1];
+          w_i[3 * i + 2] += dvar * e_ij[2];
+        }
+      }
+      forward(w_i, 3, nlocal, nghost, owner);  // :547-548
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < nlocal; ++i) {
+        if (!(mask[i] & groupbit)) continue;
+        int itype = type[i];
+        if (!(rho_i[i] > 0)) continue;
+        double alpha_i = beta.get_alpha(type_map_beta[itype - 1], rho_i[i]);
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+          int jj = nbr[k];
+          int jtype = type[jj];
+          if (!(mask[jj] & groupbit)) continue;
+          double e_ij[3];
+          double e_r_sq = dist_sq(&x[3 * jj], &x[3 * i], e_ij);
+          if (e_r_sq >= r_cutoff_sq || !(rho_i[jj] > 0)) continue;
+          double alpha_j = beta.get_alpha(type_map_beta[jtype - 1], rho_i[jj]);
+          double v_rho_ji = beta.get_rho_r_sq(type_map_beta[jtype - 1], e_r_sq);
+          double var1 = alpha_i * v_rho_ji * dot(e_ij, &w_i[3 * i]) / (rho_i[i] * e_r_sq);
+          double v_rho_ij = beta.get_rho_r_sq(type_map_beta[itype - 1], e_r_sq);
+          double var2 = alpha_j * v_rho_ij * dot(e_ij, &w_i[3 * jj]) / (rho_i[jj] * e_r_sq);
+          double dvar = var1 - var2;
+          const double f_ij[3] = {dvar * e_ij[0], dvar * e_ij[1], dvar * e_ij[2]};
+          f_EPH[3 * i + 0] -= f_ij[0];
+          f_EPH[3 * i + 1] -= f_ij[1];
+          f_EPH[3 * i + 2] -= f_ij[2];
+          if (!(eph_flag & A_NOFRICTION)) {
+            dE[i] += 0.5 * f_ij[0] * (v[3 * i + 0] - v[3 * jj + 0]) * l_dt;
+            dE[i] += 0.5 * f_ij[1] * (v[3 * i + 1] - v[3 * jj + 1]) * l_dt;
+            dE[i] += 0.5 * f_ij[2] * (v[3 * i + 2] - v[3 * jj + 2]) * l_dt;
+          }
+        }
+      }
+    }
+    if (eph_flag & A_RANDOM) {
+#pragma omp parallel for schedule(static)
+      for (int i = 0; i < nlocal; ++i) {
+        if (!(mask[i] & groupbit)) continue;
+        int itype = type[i];
+        if (!(rho_i[i] > 0)) continue;
+        double alpha_i = beta.get_alpha(type_map_beta[itype - 1], rho_i[i]);
+        double v_Ti = std::sqrt(T_of(itype, E0[i]));
+        for (int k = off[i]; k < off[i + 1]; ++k) {
+          int jj = nbr[k];
+          int jtype = type[jj];
+          if (!(mask[jj] & groupbit)) continue;
+          double e_ij[3];
+          double e_r_sq = dist_sq(&x[3 * jj], &x[3 * i], e_ij);
+          if ((e_r_sq >= r_cutoff_sq) || !(rho_i[jj] > 0)) continue;
+          double alpha_j = beta.get_alpha(type_map_beta[jtype - 1], rho_i[jj]);
+          double v_Tj = std::sqrt(T_of(jtype, E0[jj]));
+          double v_rho_ji = beta.get_rho_r_sq(type_map_beta[jtype - 1], e_r_sq);
+          double var1 = v_Ti * alpha_i * v_rho_ji * dot(e_ij, &xi_i[3 * i]) / (rho_i[i] * e_r_sq);
+          double v_rho_ij = beta.get_rho_r_sq(type_map_beta[itype - 1], e_r_sq);
+          double var2 = v_Tj * alpha_j * v_rho_ij * dot(e_ij, &xi_i[3 * jj]) / (rho_i[jj] * e_r_sq);
+          const double dvar = eta_factor * (var1 - var2);
+          const double f_ij[3] = {dvar * e_ij[0], dvar * e_ij[1], dvar * e_ij[2]};
+          f_RNG[3 * i + 0] += f_ij[0];
+          f_RNG[3 * i + 1] += f_ij[1];
+          f_RNG[3 * i + 2] += f_ij[2];
+          if (!(eph_flag & A_NORANDOM)) {
+            dE[i] -= 0.5 * f_ij[0] * (v[3 * i + 0] - v[3 * jj + 0]) * l_dt;
+            dE[i] -= 0.5 * f_ij[1] * (v[3 * i + 1] - v[3 * jj + 1]) * l_dt;
+            dE[i] -= 0.5 * f_ij[2] * (v[3 * i + 2] - v[3 * jj + 2]) * l_dt;
+          }
+        }
+      }
+    }
+  }
+
+  // :660-762
+  void heat_solve(int nlocal, int nghost, const double* x, const int* type, const int* mask,
+                  const int* off, const int* nbr, const int* owner) {
+    int loops = inner_loops > 0 ? inner_loops : 1;
+    double scaling = 1.0 / static_cast<double>(loops);
+    double dt = dtv * scaling;
+    for (int l = 0; l < loops; ++l) {
+      for (int j = 0; j < nlocal; ++j)
+        if (mask[j] & groupbit) {
+          E0[j] += dE[j] * scaling;
+          if (E0[j] < 0.0) E0[j] = 0.0;
+        }
+      forward(E0, 1, nlocal, nghost, owner);  // state EI, :690-691
+#pragma omp parallel for schedule(static)
+      for (int j = 0; j < nlocal; ++j) {
+        E1[j] = E0[j];
+        if (!(mask[j] & groupbit)) continue;
+        int jtype = type[j];
+        double l_dE_j = 0.;
+        double l_T_j = T_of(jtype, E0[j]);
+        double l_K_j = kappa.K_T[kmap(jtype)](l_T_j);
+        const double rho_j = rho_a_i[j];
+        const double rho_j_inv = 1. / rho_a_i[j];
+        for (int k = off[j]; k < off[j + 1]; ++k) {
+          int kk = nbr[k];
+          int ktype = type[kk];
+          if (!(mask[kk] & groupbit)) continue;
+          const double rho_k = rho_a_i[kk];
+          const double rho_k_inv = 1. / rho_a_i[kk];
+          double l_T_k = T_of(ktype, E0[kk]);
+          double l_K_k = kappa.K_T[kmap(ktype)](l_T_k);
+          const double l_K = 0.5 * (l_K_j + l_K_k);
+          const double v_dT = l_T_k - l_T_j;
+          double e[3];
+          double e_r_sq = dist_sq(&x[3 * kk], &x[3 * j], e);
+          if (e_r_sq >= kappa.r_cutoff_sq) continue;
+          double v_rho_j = kappa.rho_r_sq[kmap(jtype)](e_r_sq);
+          double v_rho_k = kappa.rho_r_sq[kmap(ktype)](e_r_sq);
+          if (rho_j > 0.) l_dE_j += l_K * v_rho_k * rho_j_inv * v_dT;
+          if (rho_k > 0.) l_dE_j += l_K * v_rho_j * rho_k_inv * v_dT;
+        }
+        E1[j] = E0[j] + 0.5 * l_dE_j * dt;
+        if (E1[j] < 0) E1[j] = 0.0;
+      }
+      for (int j = 0; j < nlocal; ++j) E0[j] = E1[j];
+    }
+  }
+
+  // :764-824
+  bool post_force(int nlocal, int nghost, const double* x, const double* v, double* f,
+                  const int* type, const int* mask, const int* off, const int* nbr,
+                  const int* owner, const double* xi_in, const int* tag, long step) {
+    if (!initialised) initialise(nlocal, type, mask);
+    grow((size_t)nlocal + nghost);
+    std::fill_n(w_i.begin(), 3 * nlocal, 0.0);
+    std::fill_n(xi_i.begin(), 3 * nlocal, 0.0);
+    std::fill_n(f_EPH.begin(), 3 * nlocal, 0.0);
+    std::fill_n(f_RNG.begin(), 3 * nlocal, 0.0);
+    std::fill_n(dE.begin(), nlocal, 0.0);
+    forward(E0, 1, nlocal, nghost, owner);  // :778-779
+    if (eph_flag & A_RANDOM) {
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit) {
+          if (xi_in) {
+            for (int c = 0; c < 3; ++c) xi_i[3 * i + c] = xi_in[3 * i + c];
+          } else {
+            ora_xi((uint32_t)seed, (uint32_t)tag[i], (uint64_t)step, &xi_i[3 * i]);
+          }
+        }
+      forward(xi_i, 3, nlocal, nghost, owner);
+    }
+    calculate_environment(nlocal, x, type, mask, off, nbr);
+    forward(rho_i, 1, nlocal, nghost, owner);
+    forward(rho_a_i, 1, nlocal, nghost, owner);
+    force_prl(nlocal, nghost, x, v, type, mask, off, nbr, owner);
+    if ((eph_flag & A_FRICTION) && !(eph_flag & A_NOFRICTION))
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit)
+          for (int c = 0; c < 3; ++c) f[3 * i + c] += f_EPH[3 * i + c];
+    if ((eph_flag & A_RANDOM) && !(eph_flag & A_NORANDOM))
+      for (int i = 0; i < nlocal; ++i)
+        if (mask[i] & groupbit)
+          for (int c = 0; c < 3; ++c) f[3 * i + c] += f_RNG[3 * i + c];
+    nghost_last = nghost;
+    return true;
+  }
+  int nghost_last = 0;
+
+  // :379-414; the heat solve needs the list and positions of the step
+  void end_of_step(int nlocal, int nghost, const double* x, const int* type, const int* mask,
+                   const int* off, const int* nbr, const int* owner) {
+    if (!initialised) initialise(nlocal, type, mask);
+    if (eph_flag & A_HEAT) heat_solve(nlocal, nghost, x, type, mask, off, nbr, owner);
+    double E_local = 0.0, T_local = 0.0;
+    int count = 0;
+    for (int i = 0; i < nlocal; ++i)
+      if (mask[i] & groupbit) {
+        E_local += E0[i];
+        T_a[i] = T_of(type[i], E0[i]);
+        T_local += T_a[i];
+        count++;
+      }
+    if (count > 0) T_local /= static_cast<double>(count);
+    Ee = E_local;
+    Te = T_local / static_cast<double>(count > 0 ? 1 : 0);
+    populate_array(nlocal, type, mask);
+  }
+
+  // :416-448
+  void populate_array(int nlocal, const int* type, const int* mask) {
+    for (int i = 0; i < nlocal; ++i) {
+      double* a = &array[12 * (size_t)i];
+      if (mask[i] & groupbit) {
+        int itype = type[i];
+        a[0] = rho_i[i];
+        a[1] = beta.get_beta(type_map_beta[itype - 1], rho_i[i]);
+        for (int c = 0; c < 3; ++c) { a[2 + c] = f_EPH[3 * i + c]; a[5 + c] = f_RNG[3 * i + c]; }
+        a[8] = rho_a_i[i]; a[9] = E0[i]; a[10] = dE[i]; a[11] = T_a[i];
+      } else {
+        for (int c = 0; c < 12; ++c) a[c] = 0.0;
+      }
+    }
+  }
+};
+
 thread_local std::string g_err;
 
 }  // namespace
@@ -1135,6 +1546,99 @@ void ora_fix_get(void* h, int which, int n, double* out) {
 }
 void* ora_fix_fdm(void* h) { return &static_cast<OFix*>(h)->fdm; }
 int ora_fix_post_run(void* h) { auto* f = static_cast<OFix*>(h); return f->fdm.save_state(f->T_state.c_str()) ? 0 : -1; }
+
+// --- kappa + fix eph/atomic -------------------------------------------------
+void* ora_kappa_load(const char* path) {
+  auto* k = new OKappa();
+  if (!k->load(path, g_err)) { delete k; return nullptr; }
+  return k;
+}
+void ora_kappa_free(void* h) { delete static_cast<OKappa*>(h); }
+void ora_kappa_info(void* h, int* n_elem, int* n_pairs, double* r_cut, double* T_max) {
+  auto* k = static_cast<OKappa*>(h);
+  *n_elem = (int)k->n_elements; *n_pairs = (int)k->n_pairs; *r_cut = k->r_cutoff; *T_max = k->T_max;
+}
+// which: 0 rho(r), 1 rho(r^2) (out [n][4], *dx = inv_dx); 2 E(T) of an
+// element, 3 K(T) of a pair slot (out [n][2] = y, dy with 0 at the top)
+int ora_kappa_table(void* h, int index, int which, double* out, double* dx) {
+  auto* k = static_cast<OKappa*>(h);
+  if (which < 2) {
+    const OSpline& s = which == 0 ? k->rho_r[index] : k->rho_r_sq[index];
+    for (size_t i = 0; out && i < s.a.size(); ++i) {
+      out[4 * i] = s.a[i]; out[4 * i + 1] = s.b[i]; out[4 * i + 2] = s.c[i]; out[4 * i + 3] = s.d[i];
+    }
+    if (dx) *dx = s.inv_dx;
+    return (int)s.a.size();
+  }
+  const OLinear& L = which == 2 ? k->E_T[index] : k->K_T[index];
+  for (size_t i = 0; out && i < L.y.size(); ++i) {
+    out[2 * i] = L.y[i];
+    out[2 * i + 1] = i < L.dy.size() ? L.dy[i] : 0.0;
+  }
+  if (dx) *dx = L.dx;
+  return (int)L.y.size();
+}
+double ora_kappa_T_of_E(void* h, int elem, double E) { return static_cast<OKappa*>(h)->E_T[elem].reverse_lookup(E); }
+double ora_kappa_E_of_T(void* h, int elem, double T) { return static_cast<OKappa*>(h)->E_T[elem](T); }
+double ora_kappa_K(void* h, int slot, double T) { return static_cast<OKappa*>(h)->K_T[slot](T); }
+double ora_kappa_rho_r_sq(void* h, int elem, double r2) { return static_cast<OKappa*>(h)->rho_r_sq[elem](r2); }
+
+void* ora_atomic_create(int narg, const char* const* arg, int ntypes, double dt, double boltz,
+                        double ftm2v) {
+  auto* f = new OFixAtomic();
+  if (!f->create(narg, arg, ntypes, dt, boltz, ftm2v)) { g_err = f->err; delete f; return nullptr; }
+  return f;
+}
+void ora_atomic_free(void* h) { delete static_cast<OFixAtomic*>(h); }
+void ora_atomic_set_mass(void* h, int ntypes, const double* mass1) {
+  auto* f = static_cast<OFixAtomic*>(h);
+  f->mass.assign(mass1, mass1 + ntypes + 1);
+}
+void ora_atomic_set_groupbit(void* h, int gb) { static_cast<OFixAtomic*>(h)->groupbit = gb; }
+void ora_atomic_initial_integrate(void* h, int nlocal, double* x, double* v, const double* f,
+                                  const int* type, const int* mask) {
+  static_cast<OFixAtomic*>(h)->initial_integrate(nlocal, x, v, f, type, mask);
+}
+void ora_atomic_final_integrate(void* h, int nlocal, double* v, const double* f, const int* type,
+                                const int* mask) {
+  static_cast<OFixAtomic*>(h)->final_integrate(nlocal, v, f, type, mask);
+}
+int ora_atomic_post_force(void* h, int nlocal, int nghost, const double* x, const double* v, double* f,
+                          const int* type, const int* mask, const int* off, const int* nbr,
+                          const int* owner, const double* xi, const int* tag, long step) {
+  auto* a = static_cast<OFixAtomic*>(h);
+  if (!a->post_force(nlocal, nghost, x, v, f, type, mask, off, nbr, owner, xi, tag, step)) {
+    g_err = a->err;
+    return -1;
+  }
+  return 0;
+}
+void ora_atomic_end_of_step(void* h, int nlocal, int nghost, const double* x, const int* type,
+                            const int* mask, const int* off, const int* nbr, const int* owner) {
+  static_cast<OFixAtomic*>(h)->end_of_step(nlocal, nghost, x, type, mask, off, nbr, owner);
+}
+void ora_atomic_initialise(void* h, int nlocal, const int* type, const int* mask) {
+  static_cast<OFixAtomic*>(h)->initialise(nlocal, type, mask);
+}
+// per-atom electronic energies of the first n locals (test set-up)
+void ora_atomic_set_E(void* h, int n, const double* E) {
+  auto* f = static_cast<OFixAtomic*>(h);
+  f->grow((size_t)n);
+  std::copy(E, E + n, f->E0.begin());
+  f->initialised = true;
+}
+double ora_atomic_compute_vector(void* h, int i) {
+  auto* f = static_cast<OFixAtomic*>(h);
+  return i == 1 ? f->Te : f->Ee;  // fix_eph_atomic.cpp:855-864
+}
+// which: 0 rho, 1 w, 2 xi, 3 f_EPH, 4 f_RNG, 5 array [12], 6 rho_a, 7 E, 8 dE, 9 T
+void ora_atomic_get(void* h, int which, int n, double* out) {
+  auto* f = static_cast<OFixAtomic*>(h);
+  const std::vector<double>* v[] = {&f->rho_i, &f->w_i, &f->xi_i, &f->f_EPH, &f->f_RNG, &f->array,
+                                    &f->rho_a_i, &f->E0, &f->dE, &f->T_a};
+  const int width[] = {1, 3, 3, 3, 3, 12, 1, 1, 1, 1};
+  std::copy(v[which]->begin(), v[which]->begin() + (size_t)width[which] * n, out);
+}
 // threads of the per-atom loops (the deposit and the FDM solve stay serial,
 // as in the reference); returns the count in effect
 int ora_set_threads(int n) {

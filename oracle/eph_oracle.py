@@ -91,6 +91,28 @@ def lib():
             "ora_fix_rho_r": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
             "ora_fix_beta": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
             "ora_fix_zeta": (C.c_double, [C.c_void_p]),
+            "ora_kappa_load": (C.c_void_p, [C.c_char_p]),
+            "ora_kappa_free": (None, [C.c_void_p]),
+            "ora_kappa_info": (None, [C.c_void_p, ip, ip, dp, dp]),
+            "ora_kappa_table": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, dp]),
+            "ora_kappa_T_of_E": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_kappa_E_of_T": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_kappa_K": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_kappa_rho_r_sq": (C.c_double, [C.c_void_p, C.c_int, C.c_double]),
+            "ora_atomic_create": (C.c_void_p, [C.c_int, C.POINTER(C.c_char_p), C.c_int,
+                                               C.c_double, C.c_double, C.c_double]),
+            "ora_atomic_free": (None, [C.c_void_p]),
+            "ora_atomic_set_mass": (None, [C.c_void_p, C.c_int, dp]),
+            "ora_atomic_set_groupbit": (None, [C.c_void_p, C.c_int]),
+            "ora_atomic_initial_integrate": (None, [C.c_void_p, C.c_int, dp, dp, dp, ip, ip]),
+            "ora_atomic_final_integrate": (None, [C.c_void_p, C.c_int, dp, dp, ip, ip]),
+            "ora_atomic_post_force": (C.c_int, [C.c_void_p, C.c_int, C.c_int, dp, dp, dp, ip, ip,
+                                                ip, ip, ip, dp, ip, C.c_long]),
+            "ora_atomic_end_of_step": (None, [C.c_void_p, C.c_int, C.c_int, dp, ip, ip, ip, ip, ip]),
+            "ora_atomic_initialise": (None, [C.c_void_p, C.c_int, ip, ip]),
+            "ora_atomic_compute_vector": (C.c_double, [C.c_void_p, C.c_int]),
+            "ora_atomic_set_E": (None, [C.c_void_p, C.c_int, dp]),
+            "ora_atomic_get": (None, [C.c_void_p, C.c_int, C.c_int, dp]),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
@@ -318,6 +340,99 @@ class Fix:
     def __del__(self):
         if getattr(self, "h", None):
             lib().ora_fix_free(self.h)
+            self.h = None
+
+
+class Kappa:
+    """EPH_kappa restatement (eph_kappa.h:55-150)."""
+
+    def __init__(self, path):
+        self.h = lib().ora_kappa_load(path.encode())
+        if not self.h:
+            raise RuntimeError(lib().ora_last_error().decode())
+        ne, npair = C.c_int(), C.c_int()
+        rc, tm = C.c_double(), C.c_double()
+        lib().ora_kappa_info(self.h, C.byref(ne), C.byref(npair), C.byref(rc), C.byref(tm))
+        self.n_elements, self.n_pairs, self.r_cutoff, self.T_max = ne.value, npair.value, rc.value, tm.value
+
+    def table(self, index, which):
+        n = lib().ora_kappa_table(self.h, index, which, None, None)
+        w = 4 if which < 2 else 2
+        out = np.zeros(n * w)
+        dx = C.c_double()
+        lib().ora_kappa_table(self.h, index, which, _d(out), C.byref(dx))
+        return out.reshape(n, w), dx.value
+
+    def T_of_E(self, e, E): return lib().ora_kappa_T_of_E(self.h, e, E)
+    def E_of_T(self, e, T): return lib().ora_kappa_E_of_T(self.h, e, T)
+    def K(self, slot, T): return lib().ora_kappa_K(self.h, slot, T)
+    def rho_r_sq(self, e, r2): return lib().ora_kappa_rho_r_sq(self.h, e, r2)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_kappa_free(self.h)
+            self.h = None
+
+
+class Atomic:
+    """FixEPHAtomic restatement (fix_eph_atomic.cpp), one rank; the same hook
+    methods as Fix."""
+
+    def __init__(self, args, ntypes, dt, boltz, ftm2v, masses=None):
+        arr = (C.c_char_p * len(args))(*[a.encode() for a in args])
+        self.h = lib().ora_atomic_create(len(args), arr, ntypes, dt, boltz, ftm2v)
+        if not self.h:
+            raise RuntimeError(lib().ora_last_error().decode())
+        self.ntypes = ntypes
+        if masses is not None:
+            m = _f64(np.concatenate([[0.0], np.asarray(masses, dtype=np.float64)]))
+            lib().ora_atomic_set_mass(self.h, ntypes, _d(m))
+
+    def set_groupbit(self, gb): lib().ora_atomic_set_groupbit(self.h, gb)
+
+    def initialise(self, s):
+        lib().ora_atomic_initialise(self.h, s.nlocal, _i(s.type), _i(s.mask))
+
+    def initial_integrate(self, s):
+        lib().ora_atomic_initial_integrate(self.h, s.nlocal, _d(s.x), _d(s.v), _d(s.f), _i(s.type),
+                                           _i(s.mask))
+
+    def final_integrate(self, s):
+        lib().ora_atomic_final_integrate(self.h, s.nlocal, _d(s.v), _d(s.f), _i(s.type), _i(s.mask))
+
+    def post_force(self, s, xi=None, step=0):
+        xip = None
+        if xi is not None:
+            xi = _f64(xi)
+            xip = _d(xi)
+        rc = lib().ora_atomic_post_force(self.h, s.nlocal, s.nghost, _d(s.x), _d(s.v), _d(s.f),
+                                         _i(s.type), _i(s.mask), _i(s.nbr_off), _i(s.nbr_idx),
+                                         _i(s.ghost_owner), xip, _i(s.tag), step)
+        if rc != 0:
+            raise RuntimeError(lib().ora_last_error().decode())
+
+    def end_of_step(self, s, ntimestep=0):
+        lib().ora_atomic_end_of_step(self.h, s.nlocal, s.nghost, _d(s.x), _i(s.type), _i(s.mask),
+                                     _i(s.nbr_off), _i(s.nbr_idx), _i(s.ghost_owner))
+
+    def compute_vector(self, i): return lib().ora_atomic_compute_vector(self.h, i)
+
+    def set_E(self, E):
+        E = _f64(E)
+        lib().ora_atomic_set_E(self.h, len(E), _d(E))
+
+    _WHICH = {"rho": (0, 1), "w": (1, 3), "xi": (2, 3), "f_EPH": (3, 3), "f_RNG": (4, 3),
+              "array": (5, 12), "rho_a": (6, 1), "E": (7, 1), "dE": (8, 1), "T": (9, 1)}
+
+    def get(self, which, n):
+        idx, width = self._WHICH[which]
+        out = np.zeros(n * width)
+        lib().ora_atomic_get(self.h, idx, n, _d(out))
+        return out if width == 1 else out.reshape(n, width)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().ora_atomic_free(self.h)
             self.h = None
 
 

@@ -2,7 +2,8 @@
 //
 // TEST INFRASTRUCTURE ONLY.  oracle/Makefile compiles this file together with
 // the reference's own headers (eph_spline.h, eph_beta.h, eph_linear.h,
-// eph_fdm.h, read in place from the reference tree) into oracle/_ref/.
+// eph_kappa.h, eph_fdm.h, read in place from the reference tree) into
+// oracle/_ref/.
 // tests/golden/make_golden.py runs it to write the fixtures under
 // tests/golden/.  It prints exact values as C99 hex floats (%a).
 //
@@ -11,6 +12,7 @@
 //   beta <file>                               all four tables of every element
 //   beta_eval <file> <elem> <which> <xfile>   get_rho_r_sq / get_beta / get_alpha
 //   linear <dx> <yfile> <xfile>               operator() and reverse_lookup
+//   kappa <file>                              every table of an EPH_kappa
 //   fdm <gridfile> <dt> <nsteps> <every> [<efile>]
 //        runs nsteps of solve() (energy lines "x y z E" inserted before every
 //        solve), printing every cell's T (cell-centre get_T) after each
@@ -26,6 +28,7 @@
 #include "eph_spline.h"
 #include "eph_beta.h"
 #include "eph_linear.h"
+#include "eph_kappa.h"
 #include "eph_fdm.h"
 
 namespace {
@@ -51,6 +54,12 @@ struct BetaPeek : Beta {
     }
   }
 };
+
+void dump_linear(FILE* o, const EPH_Linear& L) {
+  std::fprintf(o, "%zu %a\n", L.y.size(), L.dx);
+  for (size_t i = 0; i < L.y.size(); ++i)
+    std::fprintf(o, "%a %a\n", L.y[i], i < L.dy.size() ? L.dy[i] : 0.0);
+}
 
 std::vector<double> read_values(const char* path) {
   std::vector<double> v;
@@ -82,6 +91,21 @@ int main(int argc, char** argv) {
     for (double x : read_values(argv[5])) {
       double y = which == 1 ? b.get_rho_r_sq(e, x) : which == 2 ? b.get_beta(e, x) : b.get_alpha(e, x);
       std::printf("%a\n", y);
+    }
+    return 0;
+  }
+  if (cmd == "kappa" && argc == 3) {
+    EPH_kappa<> k(argv[2]);
+    std::printf("%zu %zu %a %a %a\n", k.n_elements, k.n_pairs, k.r_cutoff, k.r_cutoff_sq, k.T_max);
+    for (size_t e = 0; e < k.n_elements; ++e) {
+      std::printf("ELEMENT %zu %s %d\n", e, k.element_name[e].c_str(), k.element_number[e]);
+      SplinePeek(k.rho_r[e]).dump(stdout);
+      SplinePeek(k.rho_r_sq[e]).dump(stdout);
+      dump_linear(stdout, k.E_T_atomic[e]);
+    }
+    for (size_t p = 0; p < k.n_pairs; ++p) {
+      std::printf("PAIR %zu\n", p);
+      dump_linear(stdout, k.K_T_atomic[p]);
     }
     return 0;
   }

@@ -8,11 +8,12 @@ available where the tests later run):
     python tests/golden/make_golden.py
 
 Every expected value here was produced by the reference's code
-(eph_spline.h, eph_beta.h, eph_linear.h, eph_fdm.h) through
+(eph_spline.h, eph_beta.h, eph_linear.h, eph_kappa.h, eph_fdm.h) through
 oracle/_ref/eph_ref_driver, printed as exact hex floats.  Inputs are either
 regenerated deterministically by the tests (synthetic .beta / grid files,
 whose sha256 is stored) or are data files the reference ships
-(Examples/Beta/*.beta, Tests/EPH_FDM/*/Out_Ref/*), copied here gzipped as data.
+(Examples/Beta/*.beta, Tests/EPH_FDM/*/Out_Ref/*,
+Tests/EPH_atomic_run/thermalisation/Cu.kappa), copied here gzipped as data.
 """
 import gzip
 import hashlib
@@ -236,10 +237,74 @@ def linear_fixture(tmp):
     print("wrote linear.npz")
 
 
+def ref_kappa(path):
+    lines = eo.ref_run("kappa", path).strip().split("\n")
+    head = lines[0].split()
+    res = {"n_elements": int(head[0]), "n_pairs": int(head[1]), "r_cutoff": float.fromhex(head[2]),
+           "r_cutoff_sq": float.fromhex(head[3]), "T_max": float.fromhex(head[4]),
+           "elements": [], "pairs": []}
+    k = 1
+    for e in range(res["n_elements"]):
+        assert lines[k].startswith("ELEMENT")
+        k += 1
+        tabs = []
+        for _ in range(3):  # rho(r), rho(r^2) splines, E(T) linear table
+            n, inv = lines[k].split()
+            n = int(n)
+            tabs.append((np.array(hexlines("\n".join(lines[k + 1:k + 1 + n]))), float.fromhex(inv)))
+            k += 1 + n
+        res["elements"].append(tabs)
+    for p in range(res["n_pairs"]):
+        assert lines[k].startswith("PAIR")
+        n, dx = lines[k + 1].split()
+        n = int(n)
+        res["pairs"].append((np.array(hexlines("\n".join(lines[k + 2:k + 2 + n]))), float.fromhex(dx)))
+        k += 2 + n
+    return res
+
+
+def kappa_fixture(name, path, full, sample_rows=64):
+    r = ref_kappa(path)
+    d = {"n_elements": r["n_elements"], "n_pairs": r["n_pairs"], "r_cutoff": r["r_cutoff"],
+         "T_max": r["T_max"], "file_sha256": file_sha(path)}
+    rng = np.random.default_rng(11)
+    tables = [(f"e{e}_t{w}", rows, inv) for e, tabs in enumerate(r["elements"])
+              for w, (rows, inv) in enumerate(tabs)]
+    tables += [(f"p{p}", rows, dx) for p, (rows, dx) in enumerate(r["pairs"])]
+    for key, rows, inv in tables:
+        d[key + "_dx"] = inv
+        d[key + "_sha256"] = sha(rows)
+        if full:
+            d[key] = rows
+        else:
+            idx = np.unique(np.concatenate([[0, 1, 2, len(rows) - 2, len(rows) - 1],
+                                            rng.integers(0, len(rows), sample_rows)]))
+            d[key + "_idx"] = idx
+            d[key + "_rows"] = rows[idx]
+    np.savez_compressed(os.path.join(HERE, name), **d)
+    print("wrote", name)
+
+
+def kappa_fixtures(tmp):
+    p3 = os.path.join(tmp, "synth3.kappa")
+    inputs.synthetic_kappa(p3, names=("A", "B", "C"), n_r=120, n_T=150)
+    kappa_fixture("kappa_synth3.npz", p3, True)
+    src = os.path.join(REF, "Tests/EPH_atomic_run/thermalisation/Cu.kappa")
+    with open(src, "rb") as fi, gzip.open(os.path.join(HERE, "Cu.kappa.gz"), "wb", 9) as fo:
+        shutil.copyfileobj(fi, fo)
+    kappa_fixture("kappa_Cu.npz", src, False)
+
+
 def main():
     if not eo.ref_available():
         sys.exit("oracle/_ref/eph_ref_driver missing: run `make -C oracle` where the reference exists")
     tmp = tempfile.mkdtemp()
+    if "--only" in sys.argv:  # e.g. --only kappa: write that family alone
+        try:
+            {"kappa": kappa_fixtures}[sys.argv[sys.argv.index("--only") + 1]](tmp)
+        finally:
+            shutil.rmtree(tmp)
+        return
     try:
         # splines: the sin case of Tests/EPH_Spline/test.cpp:13-25 and random data
         x = np.arange(1001) * 0.01
@@ -267,6 +332,7 @@ def main():
             beta_fixture("beta_" + name.replace(".beta", "") + ".npz", os.path.join(REF, src), False, tmp)
 
         linear_fixture(tmp)
+        kappa_fixtures(tmp)
         fdm_fixtures(tmp)
         outref_fixture()
         kat = {"source": "Random123 kat_vectors, philox4x32_10 (Salmon et al., SC'11)",

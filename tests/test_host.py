@@ -221,3 +221,38 @@ def test_spline_jacobian_matches_finite_differences():
             assert np.abs(fd - J[:, :, m]).max() <= 1e-6 * scale, m
     # the top knot's constant piece: only y[n-1], with weight 1
     assert J[-1, 0, -1] == 1.0 and np.count_nonzero(J[-1]) == 1
+
+
+def _kappa_file(tmp_path, name):
+    if name == "kappa_synth3.npz":
+        p = str(tmp_path / "synth3.kappa")
+        inputs.synthetic_kappa(p, names=("A", "B", "C"), n_r=120, n_T=150)
+        return p
+    p = str(tmp_path / "Cu.kappa")
+    with gzip.open(os.path.join(G, "Cu.kappa.gz"), "rb") as fi, open(p, "wb") as fo:
+        shutil.copyfileobj(fi, fo)
+    return p
+
+
+def kappa_golden_tables(d):
+    """(key, which, index) of every table in a kappa fixture"""
+    out = [(f"e{e}_t{w}", w, e) for e in range(int(d["n_elements"])) for w in range(3)]
+    return out + [(f"p{p}", 3, p) for p in range(int(d["n_pairs"]))]
+
+
+@pytest.mark.parametrize("name", ["kappa_synth3.npz", "kappa_Cu.npz"])
+def test_product_kappa_tables_match_reference_goldens(tmp_path, name):
+    """.kappa reader (eph_kappa.h:55-150) against EPH_kappa compiled from the
+    reference's header: rho(r), rho(r^2), the cumulative E(T) and the K(T)
+    slots, bit for bit"""
+    d = np.load(os.path.join(G, name))
+    path = _kappa_file(tmp_path, name)
+    with open(path, "rb") as f:
+        assert hashlib.sha256(f.read()).hexdigest() == str(d["file_sha256"])
+    ne, npair, rc, tmax = ea.kappa_info(path)
+    assert (ne, npair) == (int(d["n_elements"]), int(d["n_pairs"]))
+    assert rc == float(d["r_cutoff"]) and tmax == float(d["T_max"])
+    for key, which, index in kappa_golden_tables(d):
+        rows, dx = ea.kappa_table(path, index, which)
+        assert dx == float(d[key + "_dx"]), key
+        assert sha(rows) == str(d[key + "_sha256"]), key

@@ -554,3 +554,110 @@ def test_coloured_exp_v1_filters_velocities_and_noise(oracle, tmp_path):
         plain.post_force(sz, xi=zi)
         assert np.array_equal(col.get("f_EPH", 27), plain.get("f_EPH", 27))
         assert np.array_equal(col.get("f_RNG", 27), plain.get("f_RNG", 27))
+
+
+# ----------------------------------------------------------- fix eph/atomic
+from test_host import _kappa_file, kappa_golden_tables  # noqa: E402
+
+
+@pytest.mark.parametrize("name", ["kappa_synth3.npz", "kappa_Cu.npz"])
+def test_kappa_tables_golden_bitwise(oracle, tmp_path, name):
+    d = np.load(os.path.join(G, name))
+    k = oracle.Kappa(_kappa_file(tmp_path, name))
+    assert (k.n_elements, k.n_pairs) == (int(d["n_elements"]), int(d["n_pairs"]))
+    for key, which, index in kappa_golden_tables(d):
+        rows, dx = k.table(index, which)
+        assert dx == float(d[key + "_dx"]), key
+        assert sha(rows) == str(d[key + "_sha256"]), key
+        if key in d:
+            assert np.array_equal(rows, d[key]), key
+
+
+def make_atomic(oracle, tmp_path, species=1, flags="7", T_e=300.0, dt=1e-3, inner_loops=0,
+                kappa_kw=None):
+    names = ["Cu"] if species == 1 else ["A", "B", "C"][:species]
+    pb = str(tmp_path / f"a{species}.beta")
+    inputs.synthetic_beta(pb, amplitudes=[2.0 + 0.1 * e for e in range(species)],
+                          prefactors=[0.02 + 0.002 * e for e in range(species)], names=names)
+    pk = str(tmp_path / f"a{species}.kappa")
+    inputs.synthetic_kappa(pk, names=names, **(kappa_kw or {}))
+    args = inputs.atomic_args(pb, pk, names, flags=flags, T_e=T_e, inner_loops=inner_loops)
+    fix = oracle.Atomic(args, species, dt, inputs.BOLTZ, inputs.FTM2V,
+                        masses=[inputs.MASS_CU] * species)
+    plain = oracle.Fix(inputs.fix_args(pb, names, flags="3", grid=(2, 2, 2)), species,
+                       [0, 0, 0], [9.0] * 3, dt, inputs.BOLTZ, inputs.FTM2V,
+                       masses=[inputs.MASS_CU] * species)
+    return fix, plain, pk
+
+
+def test_atomic_initial_energy_from_T(oracle, tmp_path):
+    """E_i = E(T_e) of the atom's element at set-up (fix_eph_atomic.cpp:242-268);
+    Ee their sum, Te the mean of T(E_i)"""
+    L = 9.0
+    x, v, t = random_cluster(27, L, 41, species=3)
+    s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+    fix, _, pk = make_atomic(oracle, tmp_path, species=3, T_e=450.0)
+    fix.initialise(s)
+    k = oracle.Kappa(pk)
+    E = fix.get("E", 27)
+    ref = np.array([k.E_of_T(t[i] - 1, 450.0) for i in range(27)])
+    assert np.array_equal(E, ref)
+    # the cumulative table sums C(T_k) dT for k = 1..n (eph_kappa.h:132-137), so at a
+    # knot E(T) = c0 T + c1 T (T + dT) / 2 for C(T) = c0 + c1 T
+    assert np.allclose(E, 3e-6 * 450.0 + 1e-8 * 450.0 * (450.0 + 10.0) / 2, rtol=1e-12)
+    assert fix.compute_vector(0) == sum(ref.tolist())
+    assert abs(fix.compute_vector(1) - 450.0) < 1e-9
+
+
+def test_atomic_forces_equal_dense_W_operator(oracle, tmp_path):
+    """friction and random forces of eph/atomic (fix_eph_atomic.cpp:492-658):
+    f_EPH = -W^T W v, f_RNG = eta W^T (sqrt(T) xi) with per-atom T"""
+    L = 9.0
+    x, v, t = random_cluster(27, L, 43)
+    s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+    fix, plain, _ = make_atomic(oracle, tmp_path, flags="3")
+    s.type[:] = 1
+    E = np.linspace(1e-4, 3e-3, 27)  # different T per atom
+    fix.set_E(E)
+    xi = np.random.default_rng(9).normal(size=(27, 3))
+    fix.post_force(s, xi=xi)
+    plain.post_force(s, xi=xi)
+    W = W_matrix(plain, s)
+    fE = fix.get("f_EPH", 27)
+    ref = -(W.T @ (W @ v.ravel()))
+    assert np.allclose(fE.ravel(), ref, rtol=0, atol=1e-12 * np.abs(ref).max())
+    k = oracle.Kappa(str(tmp_path / "a1.kappa"))
+    sq = np.sqrt([k.T_of_E(0, e) for e in E])
+    eta = math.sqrt(2 * inputs.BOLTZ / 1e-3)
+    refR = eta * (W.T @ (sq[:, None] * xi).ravel())
+    fR = fix.get("f_RNG", 27)
+    assert np.allclose(fR.ravel(), refR, rtol=0, atol=1e-12 * np.abs(refR).max())
+    # the deposit per atom sums to the power of both forces (pair terms antisymmetric)
+    dE = fix.get("dE", 27)
+    P = ((fE + fR) * v).sum() * 1e-3
+    assert abs(dE.sum() + P) < 1e-12 * abs(P)
+
+
+def test_atomic_heat_diffusion_conserves_energy(oracle, tmp_path):
+    """heat_solve (fix_eph_atomic.cpp:660-762): the exchange between two atoms
+    is antisymmetric, so without clamping sum E is conserved; equal
+    temperatures exchange nothing"""
+    L = 9.0
+    x, v, t = random_cluster(27, L, 45, species=3)
+    s = minisys.System(x, v, t, [0, 0, 0], [L, L, L], 6.5, None)
+    fix, _, _ = make_atomic(oracle, tmp_path, species=3, flags="4", inner_loops=3)
+    E = np.random.default_rng(3).uniform(5e-4, 4e-3, 27)
+    fix.set_E(E)
+    fix.post_force(s)
+    fix.end_of_step(s)
+    E1 = fix.get("E", 27)
+    assert not np.array_equal(E1, E)
+    assert abs(E1.sum() - E.sum()) < 1e-13 * E.sum()
+    assert fix.compute_vector(0) == sum(E1.tolist())
+    # one element, uniform energy: nothing moves
+    fix1, _, _ = make_atomic(oracle, tmp_path, species=1, flags="4")
+    s.type[:] = 1
+    fix1.set_E(np.full(27, 2e-3))
+    fix1.post_force(s)
+    fix1.end_of_step(s)
+    assert np.array_equal(fix1.get("E", 27), np.full(27, 2e-3))

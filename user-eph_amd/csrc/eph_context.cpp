@@ -1106,6 +1106,46 @@ int eph_beta_table(const char* path, int elem, int which, double* coeffs, double
   return (int)s.knots.size();
 }
 
+int eph_kappa_info(const char* path, int* n_elem, int* n_pairs, double* r_cutoff, double* T_max) {
+  if (!path) return EPH_ERR_ARGS;
+  KappaModel m;
+  std::string err;
+  if (!load_kappa(path, m, err)) {
+    g_create_error = err;
+    return err.rfind("cannot open", 0) == 0 ? EPH_ERR_FILE : EPH_ERR_FORMAT;
+  }
+  if (n_elem) *n_elem = (int)m.names.size();
+  if (n_pairs) *n_pairs = m.n_pairs;
+  if (r_cutoff) *r_cutoff = m.r_cutoff;
+  if (T_max) *T_max = m.T_max;
+  return EPH_OK;
+}
+
+int eph_kappa_table(const char* path, int index, int which, double* out, double* dx) {
+  if (!path || which < 0 || which > 3) return EPH_ERR_ARGS;
+  KappaModel m;
+  std::string err;
+  if (!load_kappa(path, m, err)) {
+    g_create_error = err;
+    return err.rfind("cannot open", 0) == 0 ? EPH_ERR_FILE : EPH_ERR_FORMAT;
+  }
+  const int lim = which == 3 ? m.n_pairs : (int)m.names.size();
+  if (index < 0 || index >= lim) return EPH_ERR_ARGS;
+  if (which < 2) {
+    const Spline& s = which == 0 ? m.rho_r[index] : m.rho_r_sq[index];
+    if (out) std::memcpy(out, s.knots.data(), s.knots.size() * sizeof(Cubic));
+    if (dx) *dx = s.inv_dx;
+    return (int)s.knots.size();
+  }
+  const LinearTable& L = which == 2 ? m.E_T[index] : m.K_T[index];
+  for (size_t i = 0; out && i < L.y.size(); ++i) {
+    out[2 * i] = L.y[i];
+    out[2 * i + 1] = i < L.dy.size() ? L.dy[i] : 0.0;
+  }
+  if (dx) *dx = L.dx;
+  return (int)L.y.size();
+}
+
 // ================================================================ grid C ABI
 int eph_grid_create(eph_grid** out, int nx, int ny, int nz, const double box[6], double T_e,
                     double C_e, double rho_e, double kappa_e, int device) {

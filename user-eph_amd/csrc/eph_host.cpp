@@ -136,6 +136,90 @@ bool load_beta(const std::string& path, BetaModel& m, std::string& err) {
   return true;
 }
 
+LinearTable make_linear(double dx, const std::vector<double>& y) {
+  LinearTable L;
+  L.dx = dx;
+  L.y = y;
+  for (size_t i = 0; i + 1 < y.size(); ++i) L.dy.push_back((y[i + 1] - y[i]) / dx);
+  return L;
+}
+
+double linear_forward(const LinearTable& L, double x) {
+  if (!(x >= 0.0)) return 0.0;
+  const double q = x / L.dx;
+  if (q >= static_cast<double>(L.y.size())) return 0.0;
+  const size_t idx = static_cast<size_t>(q);
+  const double delta = x - idx * L.dx;
+  if (idx + 1 >= L.y.size()) return L.y[idx];
+  return L.y[idx] + L.dy[idx] * delta;
+}
+
+bool load_kappa(const std::string& path, KappaModel& m, std::string& err) {
+  std::ifstream in(path);
+  if (!in.is_open()) {
+    err = "cannot open .kappa file " + path;
+    return false;
+  }
+  std::string line;
+  for (int k = 0; k < 3; ++k) std::getline(in, line);  // comment lines
+  size_t ne = 0;
+  in >> ne;
+  if (!in || ne < 1) {
+    err = "no elements found in .kappa file " + path;
+    return false;
+  }
+  std::getline(in, line);
+  std::istringstream names(line);
+  m.names.assign(ne, std::string());
+  for (size_t e = 0; e < ne; ++e) names >> m.names[e];
+  m.n_pairs = ne > 1 ? static_cast<int>((ne + 1) * (ne - 1) / 2) : 1;
+  size_t nr = 0, nT = 0;
+  double dr = 0;
+  in >> nr >> dr >> m.r_cutoff >> nT >> m.dT >> m.T_max;
+  if (!in || nr < 4 || nT < 2) {
+    err = "malformed header in .kappa file " + path;
+    return false;
+  }
+  m.n_r = (int)nr;
+  m.n_T = (int)nT;
+  m.r_cutoff_sq = m.r_cutoff * m.r_cutoff;
+  const double dr_sq = m.r_cutoff_sq / static_cast<double>(nr - 1);
+  m.numbers.assign(ne, 0);
+  m.rho_r.resize(ne); m.rho_r_sq.resize(ne); m.E_T.resize(ne); m.K_T.resize(m.n_pairs);
+  std::vector<double> r(nr), c(nT);
+  for (size_t e = 0; e < ne; ++e) {
+    int z = 0;
+    in >> z;
+    m.numbers[e] = z;
+    for (size_t j = 0; j < nr; ++j) in >> r[j];
+    if (!in) {
+      err = "truncated .kappa file " + path;
+      return false;
+    }
+    if (!build_spline(dr, r, m.rho_r[e], err)) return false;
+    for (size_t j = 0; j < nr; ++j) r[j] = m.rho_r[e](std::sqrt(j * dr_sq));
+    if (!build_spline(dr_sq, r, m.rho_r_sq[e], err)) return false;
+    for (size_t j = 0; j < nT; ++j) in >> c[j];
+    if (!in) {
+      err = "truncated .kappa file " + path;
+      return false;
+    }
+    // cumulative E(T) from C(T) (eph_kappa.h:132-137)
+    c[0] = 0.;
+    for (size_t j = 1; j < nT; ++j) c[j] = c[j - 1] + c[j] * m.dT;
+    m.E_T[e] = make_linear(m.dT, c);
+  }
+  for (int k = 0; k < m.n_pairs; ++k) {
+    for (size_t j = 0; j < nT; ++j) in >> c[j];
+    if (!in) {
+      err = "truncated .kappa file " + path;
+      return false;
+    }
+    m.K_T[k] = make_linear(m.dT, c);
+  }
+  return true;
+}
+
 void GridHost::resize(size_t a, size_t b, size_t c) {
   nx = a; ny = b; nz = c;
   const size_t n = a * b * c;

@@ -49,6 +49,26 @@ struct BetaModel {
 };
 bool load_beta(const std::string& path, BetaModel& m, std::string& err);
 
+// EPH_Linear from uniform samples y at x = i dx: forward slopes (eph_linear.h:29-40)
+LinearTable make_linear(double dx, const std::vector<double>& y);
+// EPH_Linear::operator() (eph_linear.h:42-49); the top knot, whose slope the
+// reference reads past its table, returns the knot value
+double linear_forward(const LinearTable& L, double x);
+
+// .kappa contents (eph_kappa.h:55-150) for fix eph/atomic: per element the
+// locality rho(r) and its r^2 resampling, the cumulative E(T) table built
+// from C(T); then K(T) tables, one per "pair" slot, n_pairs of them with the
+// reference's count (n + 1)(n - 1) / 2 (1 for one element).
+struct KappaModel {
+  std::vector<std::string> names;
+  std::vector<int> numbers;
+  double r_cutoff = 0, r_cutoff_sq = 0, T_max = 0, dT = 0;
+  int n_r = 0, n_T = 0, n_pairs = 0;
+  std::vector<Spline> rho_r, rho_r_sq;
+  std::vector<LinearTable> E_T, K_T;
+};
+bool load_kappa(const std::string& path, KappaModel& m, std::string& err);
+
 // Cell flags of the T_e grid (eph_fdm.h:433-437).
 enum CellFlag : int8_t { CELL_CONSTANT = 0, CELL_DYNAMIC = 1, CELL_ZERO_DERIVATIVE = 2 };
 

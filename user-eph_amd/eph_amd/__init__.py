@@ -85,6 +85,8 @@ _SIG = {
     "eph_copy_probe": (C.c_int, [C.c_longlong, C.c_int, C.c_int, dp]),
     "eph_spline_build": (C.c_int, [C.c_double, dp, C.c_int, dp]),
     "eph_beta_table": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp, dp, dp]),
+    "eph_kappa_info": (C.c_int, [C.c_char_p, ip, ip, dp, dp]),
+    "eph_kappa_table": (C.c_int, [C.c_char_p, C.c_int, C.c_int, dp, dp]),
     "eph_grid_create": (C.c_int, [C.POINTER(vp), C.c_int, C.c_int, C.c_int, dp, C.c_double,
                                   C.c_double, C.c_double, C.c_double, C.c_int]),
     "eph_grid_load": (C.c_int, [C.POINTER(vp), C.c_char_p, C.c_int]),
@@ -209,6 +211,29 @@ def beta_table(path, elem, which):
     inv, rc, rho = C.c_double(), C.c_double(), C.c_double()
     L.eph_beta_table(path.encode(), elem, which, _d(out), C.byref(inv), C.byref(rc), C.byref(rho))
     return out, inv.value, rc.value, rho.value
+
+
+def kappa_info(path):
+    """(n_elements, n_pairs, r_cutoff, T_max) of a .kappa file (host only)"""
+    L = lib()
+    ne, npair, rc, tm = C.c_int(), C.c_int(), C.c_double(), C.c_double()
+    rc_ = L.eph_kappa_info(path.encode(), C.byref(ne), C.byref(npair), C.byref(rc), C.byref(tm))
+    if rc_ < 0:
+        raise EphError(rc_, L.eph_last_error(None).decode())
+    return ne.value, npair.value, rc.value, tm.value
+
+
+def kappa_table(path, index, which):
+    """Host-built table of a .kappa file: 0 rho(r), 1 rho(r^2) ([n][4], inv_dx);
+    2 E(T) of an element, 3 K(T) of a pair slot ([n][2] = y, dy; dx)"""
+    L = lib()
+    n = L.eph_kappa_table(path.encode(), index, which, None, None)
+    if n < 0:
+        raise EphError(n, L.eph_last_error(None).decode())
+    out = np.zeros((n, 4 if which < 2 else 2))
+    dx = C.c_double()
+    L.eph_kappa_table(path.encode(), index, which, _d(out), C.byref(dx))
+    return out, dx.value
 
 
 class FixEPH:

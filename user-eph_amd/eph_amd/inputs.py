@@ -111,6 +111,44 @@ def fix_args(beta_file, elements, seed=12345, flags="7", model=4, rho_e=1.0, C_e
             str(T_freq), T_out, beta_file] + list(elements)
 
 
+def synthetic_kappa(path, names=("Cu",), numbers=None, n_r=500, r_cut=4.0, amplitudes=None,
+                    r_decay=0.9, n_T=1000, dT=10.0, c0=3e-6, c1=1e-8, k0=5e-5, k1=5e-8):
+    """.kappa file for fix eph/atomic (eph_kappa.h:55-150): per element the
+    locality rho(r) = A exp(-r / r_decay) on n_r knots up to r_cut, the
+    electronic heat capacity per atom C(T) = c0 + c1 T (eV/K) on n_T knots of
+    dT; then n_pairs = (n + 1)(n - 1) / 2 (1 for one element) conduction tables
+    K(T) = (k0 + k1 T) (1 + 0.1 k), k the slot (eV / (ps K))."""
+    n = len(names)
+    numbers = numbers or [29] * n
+    amplitudes = amplitudes or [1.0 + 0.1 * e for e in range(n)]
+    dr = r_cut / (n_r - 1)
+    r = np.arange(n_r) * dr
+    T = np.arange(n_T) * dT
+    n_pairs = (n + 1) * (n - 1) // 2 if n > 1 else 1
+    with open(path, "w") as f:
+        f.write("# synthetic kappa file\n#\n#\n")
+        f.write(f"{n} " + " ".join(names) + "\n")
+        f.write(f"{n_r} {dr!r} {r_cut!r} {n_T} {dT!r} {(n_T - 1) * dT!r}\n")
+        for e in range(n):
+            f.write(f"{numbers[e]}\n")
+            f.write("\n".join(repr(float(v)) for v in amplitudes[e] * np.exp(-r / r_decay)))
+            f.write("\n")
+            f.write("\n".join(repr(float(v)) for v in c0 + c1 * T))
+            f.write("\n")
+        for k in range(n_pairs):
+            f.write("\n".join(repr(float(v)) for v in (k0 + k1 * T) * (1.0 + 0.1 * k)))
+            f.write("\n")
+    return list(names)
+
+
+def atomic_args(beta_file, kappa_file, elements, seed=12345, flags="7", T_e=300.0,
+                T_infile="NULL", inner_loops=0, out="NULL"):
+    """`fix ID group eph/atomic seed flags T_e T_infile inner_loops out beta_file kappa_file
+    elem...` (fix_eph_atomic.cpp:59-269)."""
+    return ["e", "all", "eph/atomic", str(seed), str(flags), repr(float(T_e)), T_infile,
+            str(inner_loops), out, beta_file, kappa_file] + list(elements)
+
+
 def pka_speed(energy_eV, mass=MASS_CU):
     """|v| in A/ps of an atom of `mass` g/mol carrying `energy_eV` of kinetic energy."""
     return math.sqrt(2.0 * energy_eV / (mass * MVV2E))
