@@ -66,7 +66,12 @@ enum eph_hook {
  * "eph/coloured/exp" (fix_eph_coloured_exp.cpp: arg[5] = tau0, the applied
  * f_EPH and f_RNG are exponentially filtered) and "eph/coloured/exp/v1"
  * (fix_eph_coloured_exp_v1.cpp: arg[5] = tau0, the w sum sees filtered
- * velocities and the random sum filtered noise). */
+ * velocities and the random sum filtered noise).  "eph/atomic"
+ * (fix_eph_atomic.cpp) has its own layout: ID group eph/atomic seed flags T_e
+ * T_file inner_loops output beta_file kappa_file element... (narg >= 12);
+ * per-atom electronic energies replace the T_e grid (flags: 1 friction,
+ * 2 random, 4 heat diffusion, 8 no integration, 16/32 forces not applied),
+ * and boxlo/boxhi are not used. */
 int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
                const double boxlo[3], const double boxhi[3], double dt, double boltz,
                double ftm2v, int device);
@@ -135,14 +140,27 @@ int eph_final_integrate(eph_context* ctx, double* v, const double* f);
 int eph_end_of_step(eph_context* ctx, const double* x, const double* v, long long ntimestep);
 /* FixEPH::post_run, fix_eph.cpp:1019-1021: writes the T_e state file */
 int eph_post_run(eph_context* ctx);
-/* FixEPH::compute_vector, fix_eph.cpp:940-948: 0 -> Ee, 1 -> mean T_e */
+/* FixEPH::compute_vector, fix_eph.cpp:940-948: 0 -> Ee, 1 -> mean T_e
+ * (eph/atomic, fix_eph_atomic.cpp:855-864: sum and mean over the group's atoms) */
 int eph_compute_vector(eph_context* ctx, int i, double* out);
-/* per-atom array of fix_eph.cpp:406-428: [nlocal][8] = rho, beta(rho), f_EPH, f_RNG */
+/* per-atom array of fix_eph.cpp:406-428: [nlocal][8] = rho, beta(rho), f_EPH, f_RNG;
+ * eph/atomic (fix_eph_atomic.cpp:416-448): [nlocal][12] = the same, then rho_a,
+ * E, dE, T */
 int eph_get_peratom(eph_context* ctx, double* array);
+/* columns of that array: 8, or 12 for eph/atomic */
+int eph_peratom_cols(const eph_context* ctx);
+/* eph/atomic: the per-atom electronic energies E_i, [nlocal] by local index.
+ * The reference carries them with the atoms (pack_exchange / copy_arrays,
+ * fix_eph_atomic.cpp:950-964); a caller that reorders atoms reads them before
+ * and writes them back after eph_set_atoms.  Setting them also replaces the
+ * set-up from T_e that the first eph_set_atoms performs. */
+int eph_get_atomic_energy(eph_context* ctx, double* E);
+int eph_set_atomic_energy(eph_context* ctx, const double* E);
 /* internal per-atom state after post_force, for checking:
  * which 0 rho [nall], 1 w [nall][3], 2 xi [nall][3], 3 f_EPH [nlocal][3],
  * 4 f_RNG [nlocal][3], 5 alpha [nall], 6 active neighbours (r < r_cutoff)
- * per local atom [nlocal], 7 list entries per local atom [nlocal] */
+ * per local atom [nlocal], 7 list entries per local atom [nlocal];
+ * eph/atomic also 8 rho_a [nall], 9 E [nall], 10 dE [nlocal], 11 T [nall] */
 int eph_get_state(eph_context* ctx, int which, double* out);
 /* energy deposited in the last end_of_step (E_local of fix_eph.cpp:357-402) */
 int eph_last_energy(eph_context* ctx, double* E_local);

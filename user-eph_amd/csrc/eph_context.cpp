@@ -264,7 +264,9 @@ int eph_context::reserve_atoms(int nl, int ng) {
   fstride = st;
   nlocal = nl;
   nghost = ng;
-  // the filter states cover every atom slot a later device rebuild may fill
+  // the filter states (and the eph/atomic energies) cover every atom slot a
+  // later device rebuild may fill
+  if (atomic()) return eph::atomic_reserve(this, nall);
   return grow_filter_state(nall);
 }
 
@@ -286,6 +288,7 @@ int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, con
   EPH_HIP(this, hipMemcpyAsync(vel.p, hv.data(), nall * sizeof(double4), hipMemcpyHostToDevice, stream));
   EPH_HIP(this, hipMemcpyAsync(type.p, ty, nall * sizeof(int), hipMemcpyHostToDevice, stream));
   EPH_HIP(this, hipMemcpyAsync(mask.p, mk, nall * sizeof(int), hipMemcpyHostToDevice, stream));
+  if (atomic() && !at->ready && (rc = eph::atomic_setup_energies(this, ty, mk)) != EPH_OK) return rc;
   if (tg) {
     EPH_HIP(this, hipMemcpyAsync(tag.p, tg, nall * sizeof(int), hipMemcpyHostToDevice, stream));
   } else {
@@ -411,6 +414,7 @@ int eph_context::dev_final_integrate() {
 
 int eph_context::dev_post_force() {
   if (!have_atoms || !have_list) return fail(EPH_ERR_STATE, "atoms and neighbour list must be set");
+  if (atomic()) return eph::atomic_post_force(this);
   StepParams p = step_params();
   p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
   const AtomsView a = atoms_view();
@@ -452,6 +456,7 @@ int eph_context::dev_post_force() {
 // the force sweep runs with both force additions off, so f is not touched.
 int eph_context::autotune(int* forms, double* ms) {
   if (!have_atoms || !have_list) return fail(EPH_ERR_STATE, "atoms and neighbour list must be set");
+  if (atomic() || model != 4) return fail(EPH_ERR_UNSUPPORTED, "the sweep forms belong to the PRL model");
   StepParams p = step_params();
   p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
   const AtomsView a = atoms_view();
@@ -494,6 +499,7 @@ int eph_context::autotune(int* forms, double* ms) {
 }
 
 int eph_context::dev_end_of_step(long long ntimestep) {
+  if (atomic()) return eph::atomic_end_of_step(this);
   int rc = dev_deposit();
   if (rc != EPH_OK) return rc;
   hipEvent_t e;
@@ -541,7 +547,7 @@ int eph_context::dev_deposit() {
 int eph_context::check_flags() {
   int fl = 0, gf = 0;
   EPH_HIP(this, hipMemcpyAsync(&fl, flags_dev.p, sizeof(int), hipMemcpyDeviceToHost, stream));
-  EPH_HIP(this, hipMemcpyAsync(&gf, grid->nonfinite.p, sizeof(int), hipMemcpyDeviceToHost, stream));
+  if (grid) EPH_HIP(this, hipMemcpyAsync(&gf, grid->nonfinite.p, sizeof(int), hipMemcpyDeviceToHost, stream));
   EPH_HIP(this, hipStreamSynchronize(stream));
   if (gf & 2) return fail(EPH_ERR_NONFINITE, "T_e grid became non-finite (FDM instability)");
   return EPH_OK;
@@ -568,7 +574,9 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
     g_create_error = m;
     return code;
   };
-  if (narg < 18 || !arg) return fail(EPH_ERR_ARGS, "Illegal fix eph command: too few arguments");
+  if (!arg || narg < 3) return fail(EPH_ERR_ARGS, "Illegal fix eph command: too few arguments");
+  const bool atomic_style = arg[2] && std::strcmp(arg[2], "eph/atomic") == 0;
+  if (!atomic_style && narg < 18) return fail(EPH_ERR_ARGS, "Illegal fix eph command: too few arguments");
   if (!(dt > 0)) return fail(EPH_ERR_ARGS, "timestep must be positive");
   auto* c = new eph_context();
   auto bail = [&](int code, const std::string& m) {
@@ -583,6 +591,13 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   // (fix_eph_coloured.cpp, PRL whatever arg[5] says, :652), eph/coloured/exp
   // and eph/coloured/exp/v1 (arg[5] = tau0, fix_eph_coloured_exp.cpp:113)
   const std::string style = arg[2] ? arg[2] : "";
+  GridHost gh;
+  std::string err;
+  if (atomic_style) {
+    c->colour = 4;
+    c->at.reset(new eph::AtomicState());
+    if (int rc = eph::atomic_parse(c, narg, arg, ntypes, err)) return bail(rc, err);
+  } else {
   if (style == "eph" || style == "eph/gpu") c->colour = 0;
   else if (style == "eph/coloured") c->colour = 1;
   else if (style == "eph/coloured/exp") c->colour = 2;
@@ -606,8 +621,6 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   const double v_kappa = std::atof(arg[8]), v_Te = std::atof(arg[9]);
   const int nx = std::atoi(arg[10]), ny = std::atoi(arg[11]), nz = std::atoi(arg[12]);
   // fix_eph.cpp:134-161
-  GridHost gh;
-  std::string err;
   if (std::strcmp("NULL", arg[13]) == 0) {
     if (nx < 1 || ny < 1 || nz < 1) return bail(EPH_ERR_ARGS, "FixEPH: non-positive grid values");
     const double box[6] = {boxlo[0], boxhi[0], boxlo[1], boxhi[1], boxlo[2], boxhi[2]};
@@ -632,15 +645,18 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
       if (c->beta.names[j] == arg[17 + i]) c->type_map[i] = (int)j;
     if (c->type_map[i] == INT_MAX) return bail(EPH_ERR_ARGS, "Fix eph: elements not found in input file");
   }
+  }  // fix eph layout
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return bail(EPH_ERR_NO_DEVICE, "no HIP device");
   if (device < 0 || device >= ndev) return bail(EPH_ERR_NO_DEVICE, "HIP device ordinal out of range");
   if (hipSetDevice(device) != hipSuccess) return bail(EPH_ERR_NO_DEVICE, "hipSetDevice failed");
   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
     return bail(EPH_ERR_HIP, "stream creation failed");
-  c->grid = new eph_grid();
-  c->grid->h = gh;
-  if (int rc = grid_init_device(c->grid, device, c->stream)) return bail(rc, c->grid->err);
+  if (!c->atomic()) {
+    c->grid = new eph_grid();
+    c->grid->h = gh;
+    if (int rc = grid_init_device(c->grid, device, c->stream)) return bail(rc, c->grid->err);
+  }
   // fix_eph.cpp:199-204
   c->dt = dt;
   c->boltz = boltz;
@@ -680,6 +696,8 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   t.n_beta = nb; t.inv_drho = c->beta.alpha[0].inv_dx; t.alpha = c->t_alpha.p; t.beta = c->t_beta.p;
   t.rc2 = c->beta.r_cutoff_sq; t.rho_cut = c->beta.rho_cutoff;
   t.rho_r = c->t_rho_r.p; t.inv_dr = c->beta.rho[0].inv_dx;
+  if (c->atomic())
+    if (int rc = eph::atomic_upload(c)) return bail(rc, c->err);
   *out = c;
   return EPH_OK;
 }
@@ -718,7 +736,7 @@ int eph_reset_dt(eph_context* ctx, double dt) {
   ctx->dtv = dt;
   ctx->dtf = 0.5 * dt * ctx->ftm2v;
   if (ctx->colour >= 2) ctx->zeta = 1.0 - std::exp(-dt / ctx->tau0);  // fix_eph_coloured_exp.cpp:685
-  ctx->grid->h.dt = dt;
+  if (ctx->grid) ctx->grid->h.dt = dt;
   return EPH_OK;
 }
 
@@ -894,6 +912,7 @@ int eph_end_of_step(eph_context* ctx, const double* x, const double* v, long lon
 
 int eph_post_run(eph_context* ctx) {
   if (!ctx) return EPH_ERR_ARGS;
+  if (ctx->atomic()) return EPH_OK;  // fix_eph_atomic.cpp:946-948 writes nothing
   std::vector<double> T;
   int rc;
   if ((rc = ctx->grid->sync_params_from_device())) return ctx->fail(rc, ctx->grid->err);
@@ -904,6 +923,10 @@ int eph_post_run(eph_context* ctx) {
 
 int eph_compute_vector(eph_context* ctx, int i, double* out) {
   if (!ctx || !out) return EPH_ERR_ARGS;
+  if (ctx->atomic()) {  // fix_eph_atomic.cpp:855-864
+    *out = i == 1 ? ctx->at->Te : ctx->at->Ee;
+    return EPH_OK;
+  }
   if (i == 1) return eph_grid_T_mean(ctx->grid, out);
   EPH_HIP(ctx, hipMemcpyAsync(out, ctx->scal.p + 1, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
   EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
@@ -929,11 +952,45 @@ int eph_get_peratom(eph_context* ctx, double* array) {
   if (!ctx || !array) return EPH_ERR_ARGS;
   if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
   if (ctx->nlocal == 0) return EPH_OK;
+  if (ctx->atomic()) {
+    EPH_HIP(ctx, ctx->stage.reserve(std::max<size_t>(ctx->stage.n, 12 * (size_t)ctx->nlocal)));
+    EPH_HIP(ctx, eph::launch_peratom_atomic(ctx->atoms_view(), ctx->tabs, ctx->work_view(),
+                                            ctx->at->view(), ctx->stage.p, ctx->stream));
+    EPH_HIP(ctx, hipMemcpyAsync(array, ctx->stage.p, 12 * (size_t)ctx->nlocal * sizeof(double),
+                                hipMemcpyDeviceToHost, ctx->stream));
+    EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+    return EPH_OK;
+  }
   EPH_HIP(ctx, ctx->stage.reserve(std::max<size_t>(ctx->stage.n, 8 * (size_t)ctx->nlocal)));
   EPH_HIP(ctx, launch_peratom(ctx->atoms_view(), ctx->tabs, ctx->work_view(), ctx->stage.p, ctx->stream));
   EPH_HIP(ctx, hipMemcpyAsync(array, ctx->stage.p, 8 * (size_t)ctx->nlocal * sizeof(double),
                               hipMemcpyDeviceToHost, ctx->stream));
   EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EPH_OK;
+}
+
+int eph_peratom_cols(const eph_context* ctx) {
+  if (!ctx) return EPH_ERR_ARGS;
+  return ctx->atomic() ? 12 : 8;
+}
+
+int eph_get_atomic_energy(eph_context* ctx, double* E) {
+  if (!ctx || !E) return EPH_ERR_ARGS;
+  if (!ctx->atomic()) return ctx->fail(EPH_ERR_STATE, "per-atom energies belong to eph/atomic");
+  if (ctx->nlocal)
+    EPH_HIP(ctx, hipMemcpyAsync(E, ctx->at->E.p, ctx->nlocal * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EPH_OK;
+}
+
+int eph_set_atomic_energy(eph_context* ctx, const double* E) {
+  if (!ctx || !E) return EPH_ERR_ARGS;
+  if (!ctx->atomic()) return ctx->fail(EPH_ERR_STATE, "per-atom energies belong to eph/atomic");
+  if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  if (ctx->nlocal)
+    EPH_HIP(ctx, hipMemcpy(ctx->at->E.p, E, ctx->nlocal * sizeof(double), hipMemcpyHostToDevice));
+  ctx->at->ready = true;
   return EPH_OK;
 }
 
@@ -963,6 +1020,14 @@ int eph_get_state(eph_context* ctx, int which, double* out) {
       EPH_HIP(ctx, hipStreamSynchronize(s));
       for (int i = 0; i < ctx->nlocal; ++i) out[i] = cnt[i];
       return EPH_OK;
+    }
+    case 8: case 9: case 10: case 11: {
+      if (!ctx->atomic()) return ctx->fail(EPH_ERR_ARGS, "per-atom energies belong to eph/atomic");
+      const eph::AtomicState& at = *ctx->at;
+      const double* src = which == 8 ? at.rho_a.p : which == 9 ? at.E.p : which == 10 ? at.dE.p : at.T.p;
+      const int n = which == 10 ? ctx->nlocal : nall;
+      if (n) EPH_HIP(ctx, hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
+      break;
     }
     default: return ctx->fail(EPH_ERR_ARGS, "unknown state selector");
   }

@@ -1,55 +1,20 @@
 // eph_context.hpp -- internal definition of the opaque handles of eph_amd.h.
 #pragma once
 
+#include <memory>
 #include <string>
 #include <vector>
 
 #include "eph_device.hpp"
+#include "eph_devarray.hpp"
 #include "eph_host.hpp"
+#include "eph_atomic.hpp"
 
 extern "C" {
 #include "eph_amd.h"
 }
 
 namespace eph {
-
-// Owning device allocation, grown on demand (never shrunk).
-template <typename T>
-struct DevArray {
-  T* p = nullptr;
-  size_t n = 0;
-  DevArray() = default;
-  DevArray(const DevArray&) = delete;
-  DevArray& operator=(const DevArray&) = delete;
-  DevArray(DevArray&& o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
-  DevArray& operator=(DevArray&& o) noexcept {
-    if (this != &o) {
-      release();
-      p = o.p; n = o.n;
-      o.p = nullptr; o.n = 0;
-    }
-    return *this;
-  }
-  ~DevArray() { release(); }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-  // ensure room for `want` elements; contents are not preserved
-  hipError_t reserve(size_t want) {
-    if (want <= n && p) return hipSuccess;
-    release();
-    size_t m = want < 64 ? 64 : want;
-    hipError_t e = hipMalloc(reinterpret_cast<void**>(&p), m * sizeof(T));
-    if (e != hipSuccess) {
-      p = nullptr;
-      return e;
-    }
-    n = m;
-    return hipMemset(p, 0, m * sizeof(T));
-  }
-};
 
 // Scratch of the ordered deposit (per item and per cell arrays).
 struct DepositBuffers {
@@ -111,8 +76,11 @@ struct eph_context {
   std::string err;
   // fix parameters (fix_eph.cpp:61-241)
   int seed = 0, flags = 0, model = 0, ntypes = 0, T_freq = 0, groupbit = 1;
-  // fix style: 0 eph, 1 eph/coloured, 2 eph/coloured/exp, 3 eph/coloured/exp/v1
+  // fix style: 0 eph, 1 eph/coloured, 2 eph/coloured/exp, 3 eph/coloured/exp/v1,
+  // 4 eph/atomic (per-atom electronic energies, no T_e grid; state in `at`)
   int colour = 0;
+  std::unique_ptr<eph::AtomicState> at;
+  bool atomic() const { return colour == 4; }
   double tau0 = 0, zeta = 0;
   std::string T_out, T_state;
   double dt = 0, boltz = 0, ftm2v = 0, eta = 0, dtv = 0, dtf = 0;

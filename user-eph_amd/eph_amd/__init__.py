@@ -66,6 +66,9 @@ _SIG = {
     "eph_post_run": (C.c_int, [vp]),
     "eph_compute_vector": (C.c_int, [vp, C.c_int, dp]),
     "eph_get_peratom": (C.c_int, [vp, dp]),
+    "eph_peratom_cols": (C.c_int, [vp]),
+    "eph_get_atomic_energy": (C.c_int, [vp, dp]),
+    "eph_set_atomic_energy": (C.c_int, [vp, dp]),
     "eph_get_state": (C.c_int, [vp, C.c_int, dp]),
     "eph_last_energy": (C.c_int, [vp, dp]),
     "eph_rho_cutoff_warning": (C.c_int, [vp]),
@@ -336,16 +339,27 @@ class FixEPH:
     def rho_cutoff_warning(self): return bool(lib().eph_rho_cutoff_warning(self.h))
 
     def peratom(self):
-        out = np.zeros((self.nlocal, 8))
+        out = np.zeros((self.nlocal, lib().eph_peratom_cols(self.h)))
         self._check(lib().eph_get_peratom(self.h, _d(out)))
         return out
 
+    def atomic_energy(self):
+        """eph/atomic: per-atom electronic energies [nlocal]"""
+        out = np.zeros(self.nlocal)
+        self._check(lib().eph_get_atomic_energy(self.h, _d(out)))
+        return out
+
+    def set_atomic_energy(self, E):
+        a = np.ascontiguousarray(E, dtype=np.float64).reshape(self.nlocal)
+        self._check(lib().eph_set_atomic_energy(self.h, _d(a)))
+
     def get(self, which):
         idx = {"rho": 0, "w": 1, "xi": 2, "f_EPH": 3, "f_RNG": 4, "alpha": 5, "nactive": 6,
-               "nlist": 7}[which]
+               "nlist": 7, "rho_a": 8, "E": 9, "dE": 10, "T": 11}[which]
         nall = self.nlocal + self.nghost
         shape = {0: (nall,), 5: (nall,), 1: (nall, 3), 2: (nall, 3), 6: (self.nlocal,),
-                 7: (self.nlocal,)}.get(idx, (self.nlocal, 3))
+                 7: (self.nlocal,), 8: (nall,), 9: (nall,), 10: (self.nlocal,),
+                 11: (nall,)}.get(idx, (self.nlocal, 3))
         out = np.zeros(shape)
         self._check(lib().eph_get_state(self.h, idx, _d(out)))
         return out
