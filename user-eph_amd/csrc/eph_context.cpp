@@ -321,7 +321,9 @@ int eph_context::set_ghost_map(int ng, const int* owner, const double* shift) {
 int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neighmask) {
   if (!have_atoms) return fail(EPH_ERR_STATE, "atoms must be set before the neighbour list");
   if (nl != nlocal) return fail(EPH_ERR_ARGS, "neighbour list size differs from nlocal");
+  if (!off) return fail(EPH_ERR_ARGS, "CSR offsets missing");
   if (off[0] != 0) return fail(EPH_ERR_ARGS, "CSR offsets must start at 0");
+  if (off[nl] > 0 && !idx) return fail(EPH_ERR_ARGS, "CSR indices missing");
   int mx = 0;
   const int nall = nlocal + nghost;
   for (int i = 0; i < nl; ++i) {
