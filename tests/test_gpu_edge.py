@@ -110,3 +110,94 @@ def test_full_size_cascade_invariants(tmp_path):
     dV = (inf["boxhi"][0] - inf["boxlo"][0]) ** 3 / cells ** 3
     assert abs(g.get("dT").sum() * dV * dt - E_local) <= 1e-10 * abs(E_local)
     assert np.isfinite(fix.compute_vector(0))
+
+
+def _fix(tmp_path, L, flags="7", grid=(2, 2, 2), beta_kw=None):
+    p = str(tmp_path / "f.beta")
+    if beta_kw:
+        names = inputs.synthetic_beta(p, **beta_kw)
+    else:
+        names = inputs.workload_beta(p, 1)
+    args = inputs.fix_args(p, names, flags=flags, grid=grid)
+    return args
+
+
+def test_init_rejects_2d_nonperiodic_triclinic(tmp_path):
+    """FixEPH::init (fix_eph.cpp:264-269) as status codes"""
+    args = _fix(tmp_path, 10.0)
+    for kw in (dict(dimension=2), dict(nonperiodic=1), dict(triclinic=1)):
+        fix = ea.FixEPH(args, 1, [0, 0, 0], [10.0] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V)
+        with pytest.raises(ea.EphError) as e:
+            fix.init(**kw)
+        assert e.value.code == -1, kw
+
+
+def test_rho_above_cutoff_zeroes_alpha_and_warns(oracle, tmp_path):
+    """rho_i > rho_cutoff: alpha = beta = 0 with the warning (eph_beta.h:171-198),
+    here with rho_cutoff = 0.999 below the lattice density 1.97"""
+    a0 = 3.615
+    x = minisys.fcc(3, a0)
+    L = 3 * a0
+    v = np.random.default_rng(5).normal(0, 3.0, x.shape)
+    sA = minisys.System(x, v, np.ones(len(x), int), [0, 0, 0], [L] * 3, 6.5, None)
+    sB = minisys.System(x, v, np.ones(len(x), int), [0, 0, 0], [L] * 3, 6.5, None)
+    args = _fix(tmp_path, L, flags="3", beta_kw=dict(drho=0.001))
+    ora = oracle.Fix(args, 1, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V,
+                     masses=[inputs.MASS_CU])
+    gpu = ea.FixEPH(args, 1, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU])
+    gpu.init()
+    gpu.set_atoms(sB)
+    assert not gpu.rho_cutoff_warning()
+    ora.post_force(sA, step=1)
+    gpu.post_force(sB, step=1)
+    n = sA.nlocal
+    assert np.all(gpu.get("rho")[:n] > 0.999)
+    assert gpu.rho_cutoff_warning()
+    assert np.all(gpu.get("alpha")[:n] == 0.0)
+    for q in ("rho", "f_EPH", "f_RNG"):
+        assert np.array_equal(gpu.get(q)[:n], ora.get(q, n)), q
+    assert np.all(gpu.get("f_EPH") == 0.0)
+
+
+def test_nonfinite_grid_is_reported(tmp_path):
+    """a non-finite T_e (an unstable stencil) is a status code, not an abort"""
+    L = 10.0
+    args = _fix(tmp_path, L, flags="7", grid=(3, 3, 3))
+    fix = ea.FixEPH(args, 1, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU])
+    fix.init()
+    x, v, t = np.array([[2.0, 2.0, 2.0], [4.5, 2.0, 2.0]]), np.ones((2, 3)), [1, 1]
+    s = minisys.System(x, v, t, [0, 0, 0], [L] * 3, 6.5, None)
+    fix.set_atoms(s)
+    g = fix.grid()
+    T = g.get("T")
+    T[4] = float("nan")
+    g.set("T", T)
+    fix.post_force(s, step=1)
+    with pytest.raises(ea.EphError) as e:
+        fix.end_of_step(s, 1)
+    assert e.value.code == -7
+
+
+def test_two_runs_are_bit_identical(tmp_path):
+    """no floating-point atomics anywhere: the same inputs give the same bits"""
+    md = ea.MD(8, ntypes=2, seed=3, T=900.0, pka_eV=500.0)
+    inf = md.info()
+    p = str(tmp_path / "r.beta")
+    names = inputs.workload_beta(p, 2)
+    args = inputs.fix_args(p, names, grid=(8, 8, 8))
+    out = []
+    for _ in range(2):
+        fix = ea.FixEPH(args, 2, inf["boxlo"], inf["boxhi"], 2.5e-4, inputs.BOLTZ, inputs.FTM2V,
+                        masses=[inputs.MASS_CU] * 2)
+        fix.init()
+        s = md.snapshot()
+        fix.set_atoms(s)
+        for n in range(1, 6):
+            fix.post_force(s, step=n)
+            fix.end_of_step(s, n)
+        out.append((s.f.copy(), fix.grid().get("T"), fix.compute_vector(0)))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
+    assert out[0][2] == out[1][2]

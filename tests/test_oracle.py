@@ -661,3 +661,35 @@ def test_atomic_heat_diffusion_conserves_energy(oracle, tmp_path):
     fix1.post_force(s)
     fix1.end_of_step(s)
     assert np.array_equal(fix1.get("E", 27), np.full(27, 2e-3))
+
+
+def test_host_code_under_address_and_ub_sanitizers(tmp_path):
+    """the oracle (every fix style, model and eph/atomic, the FDM solver) and the
+    product's .beta / .kappa / grid readers run under ASan + UBSan without a
+    report (oracle/san_driver.cpp)"""
+    import shutil
+    import subprocess
+    from drive import write_grid_fast
+    gxx = shutil.which("g++")
+    if gxx is None:
+        pytest.skip("g++ not available")
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "san_driver")
+    cmd = [gxx, "-std=c++17", "-O1", "-g", "-fopenmp", "-ffp-contract=off",
+           "-fsanitize=address,undefined", "-fno-sanitize-recover=all",
+           "-I", os.path.join(root, "user-eph_amd", "csrc"),
+           os.path.join(root, "oracle", "san_driver.cpp"),
+           os.path.join(root, "user-eph_amd", "csrc", "eph_host.cpp"), "-o", exe]
+    subprocess.run(cmd, check=True, capture_output=True, timeout=300)
+    pb = str(tmp_path / "abc.beta")
+    inputs.synthetic_beta(pb, amplitudes=(2.0, 2.1, 1.9), prefactors=(0.02, 0.022, 0.018),
+                          names=["A", "B", "C"])
+    pk = str(tmp_path / "abc.kappa")
+    inputs.synthetic_kappa(pk, names=("A", "B", "C"), n_r=200, n_T=300)
+    pg = str(tmp_path / "g.T")
+    T = 300.0 + 50.0 * np.arange(27)
+    write_grid_fast(pg, 3, (0.0, 10.4, 0.0, 10.4, 0.0, 10.4), T, C_e=3e-6, kappa=0.1, rho=1.0)
+    env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS="2")
+    r = subprocess.run([exe, pb, pk, pg], capture_output=True, text=True, timeout=300, env=env)
+    assert r.returncode == 0 and "san_driver: ok" in r.stdout, r.stderr[-3000:]
+    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
