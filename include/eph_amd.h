@@ -37,7 +37,7 @@ enum eph_status {
   EPH_ERR_HIP = -5,         /* HIP runtime error                             */
   EPH_ERR_STATE = -6,       /* call out of order (e.g. no atoms uploaded)    */
   EPH_ERR_NONFINITE = -7,   /* T_e grid became non-finite (FDM blow-up)      */
-  EPH_ERR_UNSUPPORTED = -8, /* model other than 4 (PRL) / -1 (TESTING)       */
+  EPH_ERR_UNSUPPORTED = -8, /* model 3 (PRLCM), or a feature outside a style  */
   EPH_ERR_NO_DEVICE = -9    /* no HIP device                                 */
 };
 

@@ -547,7 +547,8 @@ void ora_xi(uint32_t seed, uint32_t tag, uint64_t step, double out[3]) {
 }
 
 // ---------------------------------------------------------------------------
-// FixEPH restatement (model 4 = PRL): fix_eph.cpp:61-1021
+// FixEPH restatement (models 1, 2, 4 and TESTING; the coloured styles):
+// fix_eph.cpp:61-1021, fix_eph_coloured*.cpp
 // ---------------------------------------------------------------------------
 enum Flag { FRICTION = 0x01, RANDOM = 0x02, FDM = 0x04, NOINT = 0x08, NOFRICTION = 0x10, NORANDOM = 0x20 };
 
