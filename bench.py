@@ -85,6 +85,36 @@ def cpu_baseline(args_fix, species, snap, md, dt, nsteps, threads, first=1):
     return snap.nlocal * nsteps / t_fix, t_fix, used
 
 
+def host_path_rate(fargs, species, snap, inf, dt, nsteps):
+    """The same fix driven through the host-array hooks (x, v, f handed over
+    from host memory every call, as a LAMMPS adaptor without device-resident
+    atoms would): PCIe copies included.  Times only the four hook calls; the
+    first step warms up."""
+    import eph_amd as ea
+    from eph_amd import inputs
+    fix = ea.FixEPH(fargs, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU] * species)
+    fix.init()
+    fix.set_atoms(snap)
+    snap.f[:] = 0.0
+    t_fix = 0.0
+    for n in range(1, nsteps + 2):
+        t0 = time.perf_counter()
+        fix.initial_integrate(snap)
+        t1 = time.perf_counter()
+        snap.x[snap.nlocal:] = snap.x[snap.ghost_owner] + snap.ghost_shift
+        snap.v[snap.nlocal:] = snap.v[snap.ghost_owner]
+        t2 = time.perf_counter()
+        fix.post_force(snap, step=n)
+        fix.final_integrate(snap)
+        fix.end_of_step(snap, n)
+        t3 = time.perf_counter()
+        if n > 1:
+            t_fix += (t1 - t0) + (t3 - t2)
+    fix.close()
+    return snap.nlocal * nsteps / t_fix, t_fix
+
+
 ENS_REPLICAS, ENS_STEPS, ENS_DT = 64, 300, 2.5e-4
 
 
@@ -256,6 +286,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2, help="one-thread CPU baseline steps")
     ap.add_argument("--cpu-all-steps", type=int, default=6, help="all-thread CPU baseline steps")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--host-steps", type=int, default=3,
+                    help="steps of the host-array (PCIe-inclusive) rate, 0 to skip")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default sweep forms instead of timing both")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -296,10 +328,13 @@ def main():
                     masses=[inputs.MASS_CU] * species)
     fix.init()
     cpu = cpu1 = None
+    snap_host = None
+    if args.host_steps > 0:
+        snap_host = md.snapshot()  # for the host-array (PCIe-inclusive) rate
     if not args.no_cpu_baseline:
         # timed before the device run, while the harness lists still match snap0
         log("cpu baseline ...")
-        snap0 = md.snapshot()
+        snap0 = copy_snapshot(snap_host) if snap_host is not None else md.snapshot()
         threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or
                              (os.cpu_count() or 1)))
         if args.cpu_steps > 0:
@@ -422,6 +457,14 @@ def main():
         ms = ea.copy_probe(nbytes, 3)
         out["calibration_copy"] = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": ms,
                                    "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
+    if snap_host is not None:
+        rate, t_h = host_path_rate(fargs, species, snap_host, inf, dt, args.host_steps)
+        out["host_array_path"] = {
+            "atom_steps_per_s": rate, "steps": args.host_steps, "seconds": t_h,
+            "note": "x, v, f in host memory, copied over PCIe at every hook (eph_post_force etc.); "
+                    "not the value, which keeps the atoms resident in HBM"}
+        log(f"host-array path: {rate:.3e} atom-steps/s")
+        snap_host = None
     if args.ensemble_replicas > 0:
         fix.close()
         md.close()
