@@ -469,11 +469,10 @@ int eph_context::autotune(int* forms, double* ms) {
   StepParams pf = p;
   pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
   hipEvent_t e0 = nullptr, e1 = nullptr;
-  EPH_HIP(this, hipEventCreate(&e0));
-  EPH_HIP(this, hipEventCreate(&e1));
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
   double t[4] = {0, 0, 0, 0};
   const int nv = lds_tables_possible(tabs) ? 2 : 1;
-  hipError_t he = hipSuccess;
   for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
     if (phase == 1 && model != 4) break;
     for (int v = 0; v < nv && he == hipSuccess; ++v) {
@@ -490,8 +489,8 @@ int eph_context::autotune(int* forms, double* ms) {
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (he != hipSuccess) return fail_hip(he, "autotune");
   form_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
   form_force = (nv > 1 && model == 4 && t[3] < t[2]) ? 1 : 0;
