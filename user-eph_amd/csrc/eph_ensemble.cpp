@@ -352,30 +352,32 @@ int eph_ensemble::autotune(double* ms) {
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   StepParams pf = p;
   pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
-  hipEvent_t e0, e1;
-  EPH_HIP(this, hipEventCreate(&e0));
-  EPH_HIP(this, hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t he = hipEventCreate(&e0);
+  if (he == hipSuccess) he = hipEventCreate(&e1);
   double t[4] = {0, 0, 0, 0};
   const int nv = batch_lds_possible(sys->tabs) ? 2 : 1;
-  for (int phase = 0; phase < 2; ++phase) {
-    for (int v = 0; v < nv; ++v) {
-      for (int rep = 0; rep < 4; ++rep) {  // first launch warms up
-        EPH_HIP(this, hipEventRecord(e0, s));
-        if (phase == 0)
-          EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
-                                               step_dev.p, s, v));
-        else if (sys->model == 4)
-          EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p, nchunk, s, v));
-        EPH_HIP(this, hipEventRecord(e1, s));
-        EPH_HIP(this, hipEventSynchronize(e1));
+  for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
+    for (int v = 0; v < nv && he == hipSuccess; ++v) {
+      for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // first launch warms up
+        he = hipEventRecord(e0, s);
+        if (he == hipSuccess)
+          he = phase == 0 ? launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
+                                                   step_dev.p, s, v)
+               : sys->model == 4 ? launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p,
+                                                           nchunk, s, v)
+                                 : hipSuccess;
+        if (he == hipSuccess) he = hipEventRecord(e1, s);
+        if (he == hipSuccess) he = hipEventSynchronize(e1);
         float m = 0;
-        EPH_HIP(this, hipEventElapsedTime(&m, e0, e1));
+        if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
         if (rep > 0) t[2 * phase + v] += m / 3.0;
       }
     }
   }
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (he != hipSuccess) return fail_hip(he, "autotune");
   var_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
   var_force = (nv > 1 && t[3] < t[2]) ? 1 : 0;
   if (ms) std::copy(t, t + 4, ms);
