@@ -201,3 +201,44 @@ def test_two_runs_are_bit_identical(tmp_path):
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
     assert out[0][2] == out[1][2]
+
+
+def test_full_size_cascade_step_matches_oracle(oracle, tmp_path):
+    """the 2,048,000-atom four-species cascade with its 80^3 hot-spot grid: one
+    full fix step (integrate, post_force, integrate, end_of_step with the
+    stencil) bit for bit against the oracle, whose per-atom loops run on the
+    host's threads"""
+    import copy
+    from drive import workload_files
+    cells, species, dt = 80, 4, 1e-4
+    md = ea.MD(cells, ntypes=species, seed=2024, T=300.0, pka_eV=1.0e4)
+    inf = md.info()
+    beta, names, tin = workload_files(str(tmp_path), "c2", species, cells, 3.615, cells,
+                                      hot_spot=(3000.0, 10.0))
+    args = inputs.fix_args(beta, names, seed=4242, flags="7", grid=(cells,) * 3, T_infile=tin)
+    masses = [inputs.MASS_CU] * species
+    ora = oracle.Fix(args, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                     masses=masses)
+    gpu = ea.FixEPH(args, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                    masses=masses)
+    gpu.init()
+    sA = md.snapshot()
+    md.lj(sA)
+    sB = copy.deepcopy(sA)
+    gpu.set_atoms(sB)
+    n = sA.nlocal
+    for s_, fix in ((sA, ora), (sB, gpu)):
+        fix.initial_integrate(s_)
+        s_.x[n:] = s_.x[s_.ghost_owner] + s_.ghost_shift
+        s_.v[n:] = s_.v[s_.ghost_owner]
+        fix.post_force(s_, step=1)
+        fix.final_integrate(s_)
+        fix.end_of_step(s_, 1)
+    for q in ("rho", "w", "f_EPH", "f_RNG"):
+        g, o = gpu.get(q)[:n], ora.get(q, n)
+        assert np.array_equal(g, o), q
+    for k in ("x", "v", "f"):
+        assert np.array_equal(getattr(sB, k)[:n], getattr(sA, k)[:n]), k
+    assert np.array_equal(gpu.grid().get("T"), ora.fdm().get("T"))
+    eg, eo = gpu.compute_vector(0), ora.compute_vector(0)
+    assert abs(eg - eo) <= 1e-10 * abs(eo)
