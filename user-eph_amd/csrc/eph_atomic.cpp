@@ -126,7 +126,11 @@ int atomic_reserve(eph_context* c, size_t nall) {
   EPH_HIP(c, hipStreamSynchronize(c->stream));
   DevArray<double> e;
   EPH_HIP(c, e.reserve(want));
-  if (at->E.p && at->E.n) EPH_HIP(c, hipMemcpy(e.p, at->E.p, at->E.n * sizeof(double), hipMemcpyDeviceToDevice));
+  // on the context's stream, finished before the old buffer is released
+  if (at->E.p && at->E.n) {
+    EPH_HIP(c, hipMemcpyAsync(e.p, at->E.p, at->E.n * sizeof(double), hipMemcpyDeviceToDevice, c->stream));
+    EPH_HIP(c, hipStreamSynchronize(c->stream));
+  }
   at->E = std::move(e);
   for (DevArray<double>* a : {&at->E1, &at->dE, &at->rho_a, &at->T, &at->K, &at->st1, &at->st2})
     EPH_HIP(c, a->reserve(want));

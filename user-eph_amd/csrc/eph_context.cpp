@@ -232,7 +232,10 @@ int eph_context::grow_filter_state(size_t nall) {
   for (eph::DevArray<double4>* a : {&colA, &colB}) {
     eph::DevArray<double4> b;
     EPH_HIP(this, b.reserve(want));  // zero-filled
-    if (a->p && a->n) EPH_HIP(this, hipMemcpy(b.p, a->p, a->n * sizeof(double4), hipMemcpyDeviceToDevice));
+    if (a->p && a->n) {  // on the stream, finished before the old buffer is released
+      EPH_HIP(this, hipMemcpyAsync(b.p, a->p, a->n * sizeof(double4), hipMemcpyDeviceToDevice, stream));
+      EPH_HIP(this, hipStreamSynchronize(stream));
+    }
     *a = std::move(b);
   }
   return EPH_OK;
@@ -748,7 +751,8 @@ int eph_get_filter_state(eph_context* ctx, int which, double* out) {
   if (n == 0) return EPH_OK;
   std::vector<double4> h(n);
   const eph::DevArray<double4>& src = which == 0 ? ctx->colA : ctx->colB;
-  EPH_HIP(ctx, hipMemcpy(h.data(), src.p, n * sizeof(double4), hipMemcpyDeviceToHost));
+  EPH_HIP(ctx, hipMemcpyAsync(h.data(), src.p, n * sizeof(double4), hipMemcpyDeviceToHost, ctx->stream));
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   for (int i = 0; i < n; ++i) { out[3 * i] = h[i].x; out[3 * i + 1] = h[i].y; out[3 * i + 2] = h[i].z; }
   return EPH_OK;
 }
@@ -763,8 +767,8 @@ int eph_set_filter_state(eph_context* ctx, int which, const double* in) {
   std::vector<double4> h(n);
   for (int i = 0; i < n; ++i) h[i] = make_double4(in[3 * i], in[3 * i + 1], in[3 * i + 2], 0.0);
   eph::DevArray<double4>& dst = which == 0 ? ctx->colA : ctx->colB;
+  EPH_HIP(ctx, hipMemcpyAsync(dst.p, h.data(), n * sizeof(double4), hipMemcpyHostToDevice, ctx->stream));
   EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  EPH_HIP(ctx, hipMemcpy(dst.p, h.data(), n * sizeof(double4), hipMemcpyHostToDevice));
   return EPH_OK;
 }
 
@@ -988,9 +992,11 @@ int eph_set_atomic_energy(eph_context* ctx, const double* E) {
   if (!ctx || !E) return EPH_ERR_ARGS;
   if (!ctx->atomic()) return ctx->fail(EPH_ERR_STATE, "per-atom energies belong to eph/atomic");
   if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
-  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  if (ctx->nlocal)
-    EPH_HIP(ctx, hipMemcpy(ctx->at->E.p, E, ctx->nlocal * sizeof(double), hipMemcpyHostToDevice));
+  if (ctx->nlocal) {
+    EPH_HIP(ctx, hipMemcpyAsync(ctx->at->E.p, E, ctx->nlocal * sizeof(double), hipMemcpyHostToDevice,
+                                ctx->stream));
+    EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  }
   ctx->at->ready = true;
   return EPH_OK;
 }
@@ -1308,9 +1314,9 @@ int eph_grid_solve(eph_grid* g, int* substeps) {
   if (!g) return EPH_ERR_ARGS;
   int rc = g->solve(substeps);
   if (rc) return rc;
-  EPH_HIP(g, hipStreamSynchronize(g->stream));
   int fl = 0;
-  EPH_HIP(g, hipMemcpy(&fl, g->nonfinite.p, sizeof(int), hipMemcpyDeviceToHost));
+  EPH_HIP(g, hipMemcpyAsync(&fl, g->nonfinite.p, sizeof(int), hipMemcpyDeviceToHost, g->stream));
+  EPH_HIP(g, hipStreamSynchronize(g->stream));
   if (fl & 2) return g->fail(EPH_ERR_NONFINITE, "T_e grid became non-finite (FDM instability)");
   return EPH_OK;
 }

@@ -41,7 +41,12 @@ struct DevArray {
       return e;
     }
     n = m;
-    return hipMemset(p, 0, m * sizeof(T));
+    // The clearing runs on the null stream, which is not ordered with the
+    // library's non-blocking streams: it must be finished before the caller
+    // enqueues writes of its own to the buffer, or it can overwrite them.
+    e = hipMemset(p, 0, m * sizeof(T));
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(nullptr);
   }
 };
 
