@@ -29,6 +29,7 @@ sys.path.insert(0, os.path.join(ROOT, "user-eph_amd"))
 
 METRIC = "fix-eph atom-steps/s, 2M atoms FP64 (% HBM roofline); ensemble replica-steps/s"
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6  # MI355X dense FP64 vector
 
 
 def log(*a):
@@ -391,14 +392,23 @@ def main():
     # the roofline line reports the one with the longer average launch
     prl_ms, prl_n = rep.get("prl_forces", (float("nan"), 1))
     dw_ms, dw_n = rep.get("density_w", (float("nan"), 1))
+    # FP64 operations of the reference's expressions (+ - * / each one):
+    # every list entry 8 (e, r^2); an active entry 7 more in the density
+    # loop (spline, sum) and 36 in the w loop; a force pair 3 + 5 + 6 per
+    # spline (two with several species) + 25 friction + 25 random
+    fpair = 3 + 5 + 6 * (2 if species > 1 else 1) + 50
     sweeps = {
-        "k_prl_forces": {"bytes_per_atom": 200.0 + 4.0 * nact, "avg_launch_ms": prl_ms / max(prl_n, 1)},
+        "k_prl_forces": {"bytes_per_atom": 200.0 + 4.0 * nact, "flops_per_atom": fpair * nact,
+                         "avg_launch_ms": prl_ms / max(prl_n, 1)},
         "k_density_w": {"bytes_per_atom": 132.0 + 4.0 * nlist + 4.0 * nact,
+                        "flops_per_atom": 8.0 * nlist + 43.0 * nact,
                         "avg_launch_ms": dw_ms / max(dw_n, 1)},
     }
     for v in sweeps.values():
         v["achieved_GBs"] = v["bytes_per_atom"] * N / (v["avg_launch_ms"] * 1e-3) / 1e9
         v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+        v["fp64_TFLOPs"] = v["flops_per_atom"] * N / (v["avg_launch_ms"] * 1e-3) / 1e12
+        v["fp64_frac"] = v["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
     dom = max(sweeps, key=lambda k: sweeps[k]["avg_launch_ms"])
     achieved = sweeps[dom]["achieved_GBs"]
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rep.items()}
@@ -437,6 +447,9 @@ def main():
             "traffic": None,
             "bytes_per_atom": sweeps[dom]["bytes_per_atom"],
             "avg_launch_ms": sweeps[dom]["avg_launch_ms"],
+            "fp64": {"achieved": sweeps[dom]["fp64_TFLOPs"], "peak": FP64_PEAK_TFLOPS,
+                     "unit": "TFLOP/s", "frac": sweeps[dom]["fp64_frac"],
+                     "flops_per_atom": sweeps[dom]["flops_per_atom"]},
             "sweeps": sweeps,
         },
         "cpu_baseline": None,
