@@ -287,17 +287,17 @@ int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, con
   }
   int rc = reserve_atoms(nl, ng);
   if (rc != EPH_OK) return rc;
-  EPH_HIP(this, hipMemcpyAsync(pos.p, hp.data(), nall * sizeof(double4), hipMemcpyHostToDevice, stream));
-  EPH_HIP(this, hipMemcpyAsync(vel.p, hv.data(), nall * sizeof(double4), hipMemcpyHostToDevice, stream));
-  EPH_HIP(this, hipMemcpyAsync(type.p, ty, nall * sizeof(int), hipMemcpyHostToDevice, stream));
-  EPH_HIP(this, hipMemcpyAsync(mask.p, mk, nall * sizeof(int), hipMemcpyHostToDevice, stream));
+  if ((rc = h2d(pos.p, hp.data(), nall * sizeof(double4)))) return rc;
+  if ((rc = h2d(vel.p, hv.data(), nall * sizeof(double4)))) return rc;
+  if ((rc = h2d(type.p, ty, nall * sizeof(int)))) return rc;
+  if ((rc = h2d(mask.p, mk, nall * sizeof(int)))) return rc;
   if (atomic() && !at->ready && (rc = eph::atomic_setup_energies(this, ty, mk)) != EPH_OK) return rc;
   if (tg) {
-    EPH_HIP(this, hipMemcpyAsync(tag.p, tg, nall * sizeof(int), hipMemcpyHostToDevice, stream));
+    if ((rc = h2d(tag.p, tg, nall * sizeof(int)))) return rc;
   } else {
     std::vector<int> t(nall);
     for (size_t i = 0; i < nall; ++i) t[i] = (int)(i < (size_t)nl ? i + 1 : 0);
-    EPH_HIP(this, hipMemcpyAsync(tag.p, t.data(), nall * sizeof(int), hipMemcpyHostToDevice, stream));
+    if ((rc = h2d(tag.p, t.data(), nall * sizeof(int)))) return rc;
   }
   EPH_HIP(this, hipStreamSynchronize(stream));
   have_atoms = true;
@@ -345,7 +345,9 @@ int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neigh
   EPH_HIP(this, csr_off.reserve(nl + 1));
   EPH_HIP(this, csr_idx.reserve(std::max<long>(nnz, 1)));
   EPH_HIP(this, hipMemcpyAsync(csr_off.p, off, (nl + 1) * sizeof(int), hipMemcpyHostToDevice, stream));
-  if (nnz) EPH_HIP(this, hipMemcpyAsync(csr_idx.p, idx, nnz * sizeof(int), hipMemcpyHostToDevice, stream));
+  if (nnz) {
+    if (int rc = h2d(csr_idx.p, idx, nnz * sizeof(int))) return rc;
+  }
   EPH_HIP(this, launch_csr_to_ell(nl, csr_off.p, csr_idx.p, neighmask, nbr.p, nnbr.p, stride, stream));
   EPH_HIP(this, hipStreamSynchronize(stream));
   have_list = true;
@@ -823,8 +825,7 @@ int eph_set_noise(eph_context* ctx, const double* xi) {
     return EPH_OK;
   }
   if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
-  EPH_HIP(ctx, hipMemcpyAsync(ctx->xi_in.p, xi, 3 * (size_t)ctx->nlocal * sizeof(double),
-                              hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = ctx->h2d(ctx->xi_in.p, xi, 3 * (size_t)ctx->nlocal * sizeof(double))) return rc;
   ctx->xi_mode_next = 1;
   return EPH_OK;
 }
@@ -836,9 +837,35 @@ int eph_set_timestep(eph_context* ctx, long long ntimestep) {
 }
 
 // ---------------------------------------------------------- host-array hooks
+namespace {
+constexpr size_t kStageChunk = size_t(8) << 20;  // bytes per pinned staging chunk
+}
+
+int eph_context::h2d(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return EPH_OK;
+  if (!stager) {
+    auto s = std::make_unique<HostStager>();
+    EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
+    stager = std::move(s);
+  }
+  EPH_HIP(this, stager->to_device(dst, src, bytes, stream));
+  return EPH_OK;
+}
+
+int eph_context::d2h(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return EPH_OK;
+  if (!stager) {
+    auto s = std::make_unique<HostStager>();
+    EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
+    stager = std::move(s);
+  }
+  EPH_HIP(this, stager->to_host(dst, src, bytes, stream));
+  return EPH_OK;
+}
+
 static int upload_rows(eph_context* c, const double* rows, int n, double4* dst) {
   if (n == 0) return EPH_OK;
-  EPH_HIP(c, hipMemcpyAsync(c->stage.p, rows, 3 * (size_t)n * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = c->h2d(c->stage.p, rows, 3 * (size_t)n * sizeof(double))) return rc;
   EPH_HIP(c, launch_rows_into_d4(c->stage.p, n, dst, c->stream));
   return EPH_OK;
 }
@@ -846,13 +873,12 @@ static int upload_rows(eph_context* c, const double* rows, int n, double4* dst) 
 static int download_rows(eph_context* c, const double4* src, int n, double* rows, int stride4 = 1) {
   if (n == 0) return EPH_OK;
   EPH_HIP(c, launch_d4_into_rows(src, n, c->stage.p, c->stream, stride4));
-  EPH_HIP(c, hipMemcpyAsync(rows, c->stage.p, 3 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost, c->stream));
-  return EPH_OK;
+  return c->d2h(rows, c->stage.p, 3 * (size_t)n * sizeof(double));
 }
 
 static int upload_force(eph_context* c, const double* f) {
   if (c->nlocal == 0) return EPH_OK;
-  EPH_HIP(c, hipMemcpyAsync(c->stage.p, f, 3 * (size_t)c->nlocal * sizeof(double), hipMemcpyHostToDevice, c->stream));
+  if (int rc = c->h2d(c->stage.p, f, 3 * (size_t)c->nlocal * sizeof(double))) return rc;
   EPH_HIP(c, launch_rows_into_soa(c->stage.p, c->nlocal, c->f.p, c->fstride, c->stream));
   return EPH_OK;
 }
@@ -884,8 +910,7 @@ int eph_post_force(eph_context* ctx, const double* x, const double* v, double* f
   if ((rc = ctx->dev_post_force())) return rc;
   if (ctx->nlocal) {
     EPH_HIP(ctx, launch_soa_into_rows(ctx->f.p, ctx->nlocal, ctx->fstride, ctx->stage.p, ctx->stream));
-    EPH_HIP(ctx, hipMemcpyAsync(f, ctx->stage.p, 3 * (size_t)ctx->nlocal * sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
+    if ((rc = ctx->d2h(f, ctx->stage.p, 3 * (size_t)ctx->nlocal * sizeof(double)))) return rc;
   }
   EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ctx->check_flags();
@@ -961,17 +986,11 @@ int eph_get_peratom(eph_context* ctx, double* array) {
     EPH_HIP(ctx, ctx->stage.reserve(std::max<size_t>(ctx->stage.n, 12 * (size_t)ctx->nlocal)));
     EPH_HIP(ctx, eph::launch_peratom_atomic(ctx->atoms_view(), ctx->tabs, ctx->work_view(),
                                             ctx->at->view(), ctx->stage.p, ctx->stream));
-    EPH_HIP(ctx, hipMemcpyAsync(array, ctx->stage.p, 12 * (size_t)ctx->nlocal * sizeof(double),
-                                hipMemcpyDeviceToHost, ctx->stream));
-    EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-    return EPH_OK;
+    return ctx->d2h(array, ctx->stage.p, 12 * (size_t)ctx->nlocal * sizeof(double));
   }
   EPH_HIP(ctx, ctx->stage.reserve(std::max<size_t>(ctx->stage.n, 8 * (size_t)ctx->nlocal)));
   EPH_HIP(ctx, launch_peratom(ctx->atoms_view(), ctx->tabs, ctx->work_view(), ctx->stage.p, ctx->stream));
-  EPH_HIP(ctx, hipMemcpyAsync(array, ctx->stage.p, 8 * (size_t)ctx->nlocal * sizeof(double),
-                              hipMemcpyDeviceToHost, ctx->stream));
-  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
-  return EPH_OK;
+  return ctx->d2h(array, ctx->stage.p, 8 * (size_t)ctx->nlocal * sizeof(double));
 }
 
 int eph_peratom_cols(const eph_context* ctx) {

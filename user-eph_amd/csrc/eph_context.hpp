@@ -9,6 +9,7 @@
 #include "eph_devarray.hpp"
 #include "eph_host.hpp"
 #include "eph_atomic.hpp"
+#include "eph_stage.hpp"
 
 extern "C" {
 #include "eph_amd.h"
@@ -110,6 +111,9 @@ struct eph_context {
   int form_density = -1, form_force = -1;
   long long step = 0;
   double* pinned = nullptr;  // small pinned host scratch
+  // copies of the host-array hooks (pinned chunks, worker threads), made on
+  // first use
+  std::unique_ptr<eph::HostStager> stager;
   // optional per-kernel timing
   bool timing = false;
   struct TimedLaunch { const char* name; hipEvent_t a, b; };
@@ -139,6 +143,10 @@ struct eph_context {
   int autotune(int* forms, double* ms);
   int grow_filter_state(size_t nall);
   int check_flags();
+  // caller's host memory <-> device buffers on the context's stream: h2d
+  // returns once src may be reused, d2h once dst holds the data
+  int h2d(void* dst, const void* src, size_t bytes);
+  int d2h(void* dst, const void* src, size_t bytes);
   // timing helpers
   hipEvent_t take_event();
   void timed_begin(const char* name, hipEvent_t* a);
