@@ -97,6 +97,10 @@ def host_path_rate(fargs, species, snap, inf, dt, nsteps):
                     masses=[inputs.MASS_CU] * species)
     fix.init()
     fix.set_atoms(snap)
+    # what a reneighbouring step adds: atoms, ghost map and the CSR list again
+    t0 = time.perf_counter()
+    fix.set_atoms(snap)
+    t_reneigh = time.perf_counter() - t0
     snap.f[:] = 0.0
     t_fix = 0.0
     for n in range(1, nsteps + 2):
@@ -113,7 +117,7 @@ def host_path_rate(fargs, species, snap, inf, dt, nsteps):
         if n > 1:
             t_fix += (t1 - t0) + (t3 - t2)
     fix.close()
-    return snap.nlocal * nsteps / t_fix, t_fix
+    return snap.nlocal * nsteps / t_fix, t_fix, t_reneigh
 
 
 ENS_REPLICAS, ENS_STEPS, ENS_DT = 64, 300, 2.5e-4
@@ -471,9 +475,10 @@ def main():
         out["calibration_copy"] = {"bytes_read": nbytes, "bytes_written": nbytes, "avg_ms": ms,
                                    "GBs": 2 * nbytes / (ms * 1e-3) / 1e9}
     if snap_host is not None:
-        rate, t_h = host_path_rate(fargs, species, snap_host, inf, dt, args.host_steps)
+        rate, t_h, t_rn = host_path_rate(fargs, species, snap_host, inf, dt, args.host_steps)
         out["host_array_path"] = {
             "atom_steps_per_s": rate, "steps": args.host_steps, "seconds": t_h,
+            "set_atoms_and_list_ms": t_rn * 1e3,
             "note": "x, v, f in host memory, copied over PCIe at every hook (eph_post_force etc.); "
                     "not the value, which keeps the atoms resident in HBM"}
         log(f"host-array path: {rate:.3e} atom-steps/s")

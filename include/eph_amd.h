@@ -116,7 +116,9 @@ int eph_set_ghost_map(eph_context* ctx, int nghost, const int* owner, const doub
 /* NeighList numneigh/firstneigh (full list incl. ghosts) in CSR form:
  * neighbours of i are idx[off[i] .. off[i+1]), each masked with neighmask
  * (NEIGHMASK) as fix_eph.cpp:451-452 does.  Replaces transfer_neighbour_list
- * of fix_eph_gpu.cpp:241-280 / EPH_GPU::grow_neigh of lib/eph_gpu.h:67. */
+ * of fix_eph_gpu.cpp:241-280 / EPH_GPU::grow_neigh of lib/eph_gpu.h:67.
+ * A masked index outside [0, nlocal + nghost) returns EPH_ERR_ARGS and drops
+ * the list: post_force then returns EPH_ERR_STATE until a valid list is set. */
 int eph_set_neighbors(eph_context* ctx, int nlocal, const int* off, const int* idx, int neighmask);
 /* Inject the Gaussian noise xi ([nlocal][3]) used by the next post_force for
  * group atoms, instead of drawing it; NULL returns to the built-in stream.

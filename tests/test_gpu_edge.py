@@ -2,6 +2,7 @@
 
 * an empty system (no atoms) runs every hook and deposits nothing;
 * an isolated atom (rho = 0) and an isolated dimer against the oracle;
+* a list index out of range is rejected (and the list dropped);
 * the 2,048,000-atom four-species cascade lattice (fcc 80^3, 10 keV PKA, the
   benchmark's system), too large for the oracle in a test, through
   size-independent properties of one post_force / end_of_step: momentum of
@@ -158,6 +159,38 @@ def test_rho_above_cutoff_zeroes_alpha_and_warns(oracle, tmp_path):
     for q in ("rho", "f_EPH", "f_RNG"):
         assert np.array_equal(gpu.get(q)[:n], ora.get(q, n)), q
     assert np.all(gpu.get("f_EPH") == 0.0)
+
+
+def test_bad_neighbour_index_is_rejected(tmp_path):
+    """a list entry outside [0, nlocal + nghost) is an argument error (checked on
+    the device while the list is converted); the list is dropped until a valid
+    one is set, and a valid one restores the same forces"""
+    a0 = 3.615
+    x = minisys.fcc(3, a0)
+    L = 3 * a0
+    v = np.random.default_rng(6).normal(0, 3.0, x.shape)
+    s = minisys.System(x, v, np.ones(len(x), int), [0, 0, 0], [L] * 3, 6.5, None)
+    args = _fix(tmp_path, L, flags="3")
+    gpu = ea.FixEPH(args, 1, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V,
+                    masses=[inputs.MASS_CU])
+    gpu.init()
+    gpu.set_atoms(s)
+    gpu.post_force(s, step=1)
+    f_ref = gpu.get("f_EPH").copy()
+    good = s.nbr_idx.copy()
+    for bad in (s.nlocal + s.nghost, -3):
+        s.nbr_idx = good.copy()
+        s.nbr_idx[len(good) // 2] = bad
+        with pytest.raises(ea.EphError) as e:
+            gpu.set_atoms(s)
+        assert e.value.code == -1, bad
+        with pytest.raises(ea.EphError) as e:
+            gpu.post_force(s, step=1)
+        assert e.value.code == -6  # no list
+    s.nbr_idx = good
+    gpu.set_atoms(s)
+    gpu.post_force(s, step=1)
+    assert np.array_equal(gpu.get("f_EPH"), f_ref)
 
 
 def test_nonfinite_grid_is_reported(tmp_path):

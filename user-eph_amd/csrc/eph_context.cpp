@@ -277,20 +277,23 @@ int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, con
                            const int* mk, const int* tg) {
   if (nl < 0 || ng < 0 || (nl + ng > 0 && (!x || !v || !ty || !mk))) return fail(EPH_ERR_ARGS, "bad atom arrays");
   const size_t nall = (size_t)nl + ng;
-  std::vector<double4> hp(nall), hv(nall);
-  type_host.assign(ty, ty + nall);
-  for (size_t i = 0; i < nall; ++i) {
+  for (size_t i = 0; i < nall; ++i)
     if (ty[i] < 1 || ty[i] > ntypes) return fail(EPH_ERR_ARGS, "atom type out of range");
-    const int e = type_map[ty[i] - 1];
-    hp[i] = make_double4(x[3 * i], x[3 * i + 1], x[3 * i + 2], (double)e);
-    hv[i] = make_double4(v[3 * i], v[3 * i + 1], v[3 * i + 2], 0.0);
-  }
   int rc = reserve_atoms(nl, ng);
   if (rc != EPH_OK) return rc;
-  if ((rc = h2d(pos.p, hp.data(), nall * sizeof(double4)))) return rc;
-  if ((rc = h2d(vel.p, hv.data(), nall * sizeof(double4)))) return rc;
+  if (species_dev.n < (size_t)ntypes) {
+    EPH_HIP(this, species_dev.reserve(ntypes));
+    EPH_HIP(this, hipMemcpyAsync(species_dev.p, type_map.data(), ntypes * sizeof(int),
+                                 hipMemcpyHostToDevice, stream));
+  }
+  // the caller's [n][3] rows go up as they are and become (x, y, z, element)
+  // and (v, 0) records on the device
   if ((rc = h2d(type.p, ty, nall * sizeof(int)))) return rc;
   if ((rc = h2d(mask.p, mk, nall * sizeof(int)))) return rc;
+  if ((rc = h2d(stage.p, x, 3 * nall * sizeof(double)))) return rc;
+  EPH_HIP(this, launch_rows_into_d4_w(stage.p, (int)nall, type.p, species_dev.p, pos.p, stream));
+  if ((rc = h2d(stage.p, v, 3 * nall * sizeof(double)))) return rc;
+  EPH_HIP(this, launch_rows_into_d4_w(stage.p, (int)nall, type.p, nullptr, vel.p, stream));
   if (atomic() && !at->ready && (rc = eph::atomic_setup_energies(this, ty, mk)) != EPH_OK) return rc;
   if (tg) {
     if ((rc = h2d(tag.p, tg, nall * sizeof(int)))) return rc;
@@ -335,10 +338,6 @@ int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neigh
     mx = std::max(mx, n);
   }
   const long nnz = off[nl];
-  for (long k = 0; k < nnz; ++k) {
-    const int j = idx[k] & neighmask;
-    if (j < 0 || j >= nall) return fail(EPH_ERR_ARGS, "neighbour index out of range");
-  }
   maxnbr = mx;
   EPH_HIP(this, nbr.reserve((size_t)std::max(mx, 1) * stride));
   EPH_HIP(this, act.reserve((size_t)std::max(mx, 1) * stride));
@@ -348,9 +347,16 @@ int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neigh
   if (nnz) {
     if (int rc = h2d(csr_idx.p, idx, nnz * sizeof(int))) return rc;
   }
-  EPH_HIP(this, launch_csr_to_ell(nl, csr_off.p, csr_idx.p, neighmask, nbr.p, nnbr.p, stride, stream));
+  // the indices are range-checked on the device while they are converted
+  EPH_HIP(this, list_bad.reserve(1));
+  EPH_HIP(this, hipMemsetAsync(list_bad.p, 0, sizeof(int), stream));
+  EPH_HIP(this, launch_csr_to_ell(nl, csr_off.p, csr_idx.p, neighmask, nall, nbr.p, nnbr.p, stride,
+                                  list_bad.p, stream));
+  int bad = 0;
+  EPH_HIP(this, hipMemcpyAsync(&bad, list_bad.p, sizeof(int), hipMemcpyDeviceToHost, stream));
   EPH_HIP(this, hipStreamSynchronize(stream));
-  have_list = true;
+  have_list = bad == 0;
+  if (bad) return fail(EPH_ERR_ARGS, "neighbour index out of range (the list is dropped)");
   return EPH_OK;
 }
 

@@ -104,8 +104,7 @@ struct eph_context {
   const double4* rec_field(int f) const { return reinterpret_cast<const double4*>(rec.p) + f; }
   eph::DevArray<double> f, fE, fR, xi_in, stage, mass, scal;  // scal: [0] E_local [1] Ee
   eph::DevArray<int> type, mask, tag, ghost_owner, nbr, nnbr, act, nact, csr_off, csr_idx,
-      flags_dev;
-  std::vector<int> type_host;
+      flags_dev, list_bad, species_dev;  // species_dev: element of each LAMMPS type
   int xi_mode_next = 0;  // 1 once eph_set_noise injected values
   // kernel forms of the two sweeps (-1 chosen by system size; see eph_autotune)
   int form_density = -1, form_force = -1;

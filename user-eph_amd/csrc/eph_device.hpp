@@ -145,8 +145,9 @@ struct Chunk {
 
 // ---- launch wrappers (eph_kernels.hip) ----
 hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);
-hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
-                             int* nn, int stride, hipStream_t s);
+// indices outside [0, nall) set *bad (and are stored as 0)
+hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int nall,
+                             int* ell, int* nn, int stride, int* bad, hipStream_t s);
 // form: 0 one block per 256 atoms with tables read from global memory,
 // 1 persistent blocks with the tables in LDS (when they fit), -1 by size
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
@@ -192,6 +193,9 @@ hipError_t launch_peratom(const AtomsView& a, const TablesView& t, const WorkVie
                           double* out, hipStream_t s);
 // host-layout staging
 hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s);
+// whole records: w = species[type - 1] (species given) or 0
+hipError_t launch_rows_into_d4_w(const double* rows, int n, const int* type, const int* species,
+                                 double4* out, hipStream_t s);
 // in[i * stride4] for i < n (stride4 = 4 reads one field of ForceRec)
 hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s, int stride4 = 1);
 hipError_t launch_rows_into_soa(const double* rows, int n, double* soa, int stride, hipStream_t s);

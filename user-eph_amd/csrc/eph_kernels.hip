@@ -23,14 +23,23 @@ __global__ __launch_bounds__(kBlock) void k_ghost_xv(AtomsView a, const int* __r
   a.vel[a.nlocal + g] = a.vel[o];
 }
 
+// CSR -> transposed ELL; an index outside [0, nall) raises *bad and is stored
+// as 0, so that the list never addresses outside the atom arrays
 __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __restrict__ off,
                                                        const int* __restrict__ idx, int neighmask,
-                                                       int* __restrict__ ell, int* __restrict__ nn,
-                                                       int stride) {
+                                                       int nall, int* __restrict__ ell,
+                                                       int* __restrict__ nn, int stride, int* bad) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nlocal) return;
   const int b = off[i], e = off[i + 1];
-  for (int k = b; k < e; ++k) ell[(k - b) * stride + i] = idx[k] & neighmask;
+  for (int k = b; k < e; ++k) {
+    int j = idx[k] & neighmask;
+    if (j < 0 || j >= nall) {
+      atomicOr(bad, 1);
+      j = 0;
+    }
+    ell[(k - b) * stride + i] = j;
+  }
   nn[i] = e - b;
 }
 
@@ -1249,6 +1258,18 @@ __global__ __launch_bounds__(kBlock) void k_rows_into_d4(const double* __restric
   out[i] = o;
 }
 
+// rows -> (x, y, z, w) with w = element of the atom's type (species map
+// given, types already checked to lie in 1..ntypes) or 0
+__global__ __launch_bounds__(kBlock) void k_rows_into_d4_w(const double* __restrict__ rows, int n,
+                                                           const int* __restrict__ type,
+                                                           const int* __restrict__ species,
+                                                           double4* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double w = species ? static_cast<double>(species[type[i] - 1]) : 0.0;
+  out[i] = make_double4(rows[3 * i], rows[3 * i + 1], rows[3 * i + 2], w);
+}
+
 __global__ __launch_bounds__(kBlock) void k_d4_into_rows(const double4* __restrict__ in, int n,
                                                          int stride4, double* __restrict__ rows) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
@@ -1289,10 +1310,11 @@ hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* 
   return hipGetLastError();
 }
 
-hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int* ell,
-                             int* nn, int stride, hipStream_t s) {
+hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int nall,
+                             int* ell, int* nn, int stride, int* bad, hipStream_t s) {
   if (nlocal == 0) return hipSuccess;
-  k_csr_to_ell<<<nblk(nlocal), kBlock, 0, s>>>(nlocal, off, idx, neighmask, ell, nn, stride);
+  k_csr_to_ell<<<nblk(nlocal), kBlock, 0, s>>>(nlocal, off, idx, neighmask, nall, ell, nn, stride,
+                                               bad);
   return hipGetLastError();
 }
 
@@ -1588,6 +1610,12 @@ hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream
 hipError_t launch_rows_into_d4(const double* rows, int n, double4* out, hipStream_t s) {
   if (n == 0) return hipSuccess;
   k_rows_into_d4<<<nblk(n), kBlock, 0, s>>>(rows, n, out);
+  return hipGetLastError();
+}
+hipError_t launch_rows_into_d4_w(const double* rows, int n, const int* type, const int* species,
+                                 double4* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_rows_into_d4_w<<<nblk(n), kBlock, 0, s>>>(rows, n, type, species, out);
   return hipGetLastError();
 }
 hipError_t launch_d4_into_rows(const double4* in, int n, double* rows, hipStream_t s, int stride4) {

@@ -188,7 +188,9 @@ int eph_md::upload(eph_context* ctx) {
   EPH_HIP(this, hipMemcpyAsync(lj_off.p, lj.off.data(), (nlocal + 1) * sizeof(int), hipMemcpyHostToDevice, s));
   if (!lj.idx.empty())
     EPH_HIP(this, hipMemcpyAsync(lj_idx.p, lj.idx.data(), lj.idx.size() * sizeof(int), hipMemcpyHostToDevice, s));
-  EPH_HIP(this, launch_csr_to_ell(nlocal, lj_off.p, lj_idx.p, ~0, lj_nbr.p, lj_nn.p, lj_stride, s));
+  EPH_HIP(this, ctx->list_bad.reserve(1));
+  EPH_HIP(this, launch_csr_to_ell(nlocal, lj_off.p, lj_idx.p, ~0, nlocal + nghost, lj_nbr.p, lj_nn.p,
+                                  lj_stride, ctx->list_bad.p, s));
   EPH_HIP(this, xref.reserve(nlocal));
   EPH_HIP(this, hipMemcpyAsync(xref.p, ctx->pos.p, nlocal * sizeof(double4), hipMemcpyDeviceToDevice, s));
   EPH_HIP(this, partial.reserve(1024));
