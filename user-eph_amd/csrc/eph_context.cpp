@@ -845,10 +845,18 @@ int eph_set_timestep(eph_context* ctx, long long ntimestep) {
 // ---------------------------------------------------------- host-array hooks
 namespace {
 constexpr size_t kStageChunk = size_t(8) << 20;  // bytes per pinned staging chunk
+// smaller copies go through the runtime directly: a context that only ever
+// moves small arrays (the replicas of an ensemble) starts no staging threads
+constexpr size_t kStageMin = size_t(4) << 20;
 }
 
 int eph_context::h2d(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return EPH_OK;
+  if (bytes < kStageMin) {
+    EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+    EPH_HIP(this, hipStreamSynchronize(stream));
+    return EPH_OK;
+  }
   if (!stager) {
     auto s = std::make_unique<HostStager>();
     EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
@@ -860,6 +868,11 @@ int eph_context::h2d(void* dst, const void* src, size_t bytes) {
 
 int eph_context::d2h(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return EPH_OK;
+  if (bytes < kStageMin) {
+    EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+    EPH_HIP(this, hipStreamSynchronize(stream));
+    return EPH_OK;
+  }
   if (!stager) {
     auto s = std::make_unique<HostStager>();
     EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
