@@ -326,24 +326,53 @@ __global__ __launch_bounds__(kBlock) void k_adj_w(AdjStepView s, AtomsView a, Ta
 
 // ------------------------------------------------------------ alpha(rho)
 // alpha_i = a + rho (b + rho (c + rho d)) on the interval of rho_i: rho_b +=
-// alpha_b alpha', table gradient Gb[e][k] += alpha_b (1, rho, rho^2, rho^3)
+// alpha_b alpha', table gradient Gb[e][k] += alpha_b (1, rho, rho^2, rho^3).
+// The atoms of a wave mostly sit on one or two knots (rho is nearly uniform in
+// a crystal), so the table gradient is summed over the wave per distinct knot
+// first and added with one atomic per knot and wave; all lanes take part.
 __global__ __launch_bounds__(kBlock) void k_adj_alpha(AdjStepView s, AtomsView a, TablesView t,
                                                       AdjBuffers b) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  if (i >= a.nlocal || !(a.mask[i] & a.groupbit)) return;
-  const double rho = s.rec[i].rw.w;
-  const double ab = b.ab[i];
-  if (!(rho > 0.0) || rho > t.rho_cut || ab == 0.0) return;
-  const int e = t.n_elem > 1 ? static_cast<int>(s.rec[i].p.w) : 0;
-  int k = static_cast<int>(rho * t.inv_drho);
-  k = k < 0 ? 0 : (k < t.n_beta - 1 ? k : t.n_beta - 1);
-  const double4 c = t.alpha[e * t.n_beta + k];
-  b.rb[i] += ab * (c.y + rho * (2.0 * c.z + 3.0 * rho * c.w));
-  double* G = b.Gb + 4 * (static_cast<long>(e) * t.n_beta + k);
-  atomicAdd(G + 0, ab);
-  atomicAdd(G + 1, ab * rho);
-  atomicAdd(G + 2, ab * rho * rho);
-  atomicAdd(G + 3, ab * rho * rho * rho);
+  bool valid = i < a.nlocal && (a.mask[i] & a.groupbit);
+  double rho = 0.0, ab = 0.0;
+  if (valid) {
+    rho = s.rec[i].rw.w;
+    ab = b.ab[i];
+    valid = rho > 0.0 && !(rho > t.rho_cut) && ab != 0.0;
+  }
+  int key = -1;
+  if (valid) {
+    const int e = t.n_elem > 1 ? static_cast<int>(s.rec[i].p.w) : 0;
+    int k = static_cast<int>(rho * t.inv_drho);
+    k = k < 0 ? 0 : (k < t.n_beta - 1 ? k : t.n_beta - 1);
+    key = e * t.n_beta + k;
+    const double4 c = t.alpha[key];
+    b.rb[i] += ab * (c.y + rho * (2.0 * c.z + 3.0 * rho * c.w));
+  }
+  const double g0 = valid ? ab : 0.0, g1 = valid ? ab * rho : 0.0;
+  const double g2 = valid ? ab * rho * rho : 0.0, g3 = valid ? ab * rho * rho * rho : 0.0;
+  const int lane = static_cast<int>(threadIdx.x) & (warpSize - 1);
+  unsigned long long left = __ballot(valid);
+  while (left) {
+    const int lead = __ffsll(static_cast<long long>(left)) - 1;
+    const int k0 = __shfl(key, lead);
+    const bool mine = valid && key == k0;
+    double s0 = mine ? g0 : 0.0, s1 = mine ? g1 : 0.0, s2 = mine ? g2 : 0.0, s3 = mine ? g3 : 0.0;
+    for (int off = warpSize / 2; off > 0; off >>= 1) {
+      s0 += __shfl_xor(s0, off);
+      s1 += __shfl_xor(s1, off);
+      s2 += __shfl_xor(s2, off);
+      s3 += __shfl_xor(s3, off);
+    }
+    if (lane == lead) {
+      double* G = b.Gb + 4 * static_cast<long>(k0);
+      atomicAdd(G + 0, s0);
+      atomicAdd(G + 1, s1);
+      atomicAdd(G + 2, s2);
+      atomicAdd(G + 3, s3);
+    }
+    left &= ~__ballot(mine);
+  }
 }
 
 // ------------------------------------------------------- density sweep
