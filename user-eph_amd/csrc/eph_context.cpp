@@ -850,35 +850,41 @@ constexpr size_t kStageChunk = size_t(8) << 20;  // bytes per pinned staging chu
 constexpr size_t kStageMin = size_t(4) << 20;
 }
 
+// the staging path for a copy of `bytes`, or null for the runtime's own copy
+// (small copies, or pinned memory not available)
+static HostStager* stager_for(eph_context* c, size_t bytes) {
+  if (bytes < kStageMin || c->stage_failed) return nullptr;
+  if (!c->stager) {
+    auto s = std::make_unique<HostStager>();
+    if (s->init(HostStager::default_threads(), kStageChunk) != hipSuccess) {
+      (void)hipGetLastError();
+      c->stage_failed = true;
+      return nullptr;
+    }
+    c->stager = std::move(s);
+  }
+  return c->stager.get();
+}
+
 int eph_context::h2d(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return EPH_OK;
-  if (bytes < kStageMin) {
-    EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
-    EPH_HIP(this, hipStreamSynchronize(stream));
+  if (HostStager* st = stager_for(this, bytes)) {
+    EPH_HIP(this, st->to_device(dst, src, bytes, stream));
     return EPH_OK;
   }
-  if (!stager) {
-    auto s = std::make_unique<HostStager>();
-    EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
-    stager = std::move(s);
-  }
-  EPH_HIP(this, stager->to_device(dst, src, bytes, stream));
+  EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, stream));
+  EPH_HIP(this, hipStreamSynchronize(stream));
   return EPH_OK;
 }
 
 int eph_context::d2h(void* dst, const void* src, size_t bytes) {
   if (bytes == 0) return EPH_OK;
-  if (bytes < kStageMin) {
-    EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
-    EPH_HIP(this, hipStreamSynchronize(stream));
+  if (HostStager* st = stager_for(this, bytes)) {
+    EPH_HIP(this, st->to_host(dst, src, bytes, stream));
     return EPH_OK;
   }
-  if (!stager) {
-    auto s = std::make_unique<HostStager>();
-    EPH_HIP(this, s->init(HostStager::default_threads(), kStageChunk));
-    stager = std::move(s);
-  }
-  EPH_HIP(this, stager->to_host(dst, src, bytes, stream));
+  EPH_HIP(this, hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, stream));
+  EPH_HIP(this, hipStreamSynchronize(stream));
   return EPH_OK;
 }
 

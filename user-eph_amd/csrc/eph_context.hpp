@@ -113,6 +113,7 @@ struct eph_context {
   // copies of the host-array hooks (pinned chunks, worker threads), made on
   // first use
   std::unique_ptr<eph::HostStager> stager;
+  bool stage_failed = false;  // no pinned memory: the runtime copies directly
   // optional per-kernel timing
   bool timing = false;
   struct TimedLaunch { const char* name; hipEvent_t a, b; };
