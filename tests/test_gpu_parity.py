@@ -1,7 +1,7 @@
 """GPU parity: the HIP path against the oracle on identical inputs.
 
-Bar (north star): forces, rho_i and T_e within 1e-11 relative (normwise), the
-energy tally within 1e-10.  The kernels evaluate the reference's expressions
+Bar: forces, rho_i and T_e within 1e-11 relative (normwise), the energy tally
+within 1e-10.  The kernels evaluate the reference's expressions
 in its order, so most checks here also hold bit for bit and say so.
 """
 import copy
