@@ -3,6 +3,7 @@
 * an empty system (no atoms) runs every hook and deposits nothing;
 * an isolated atom (rho = 0) and an isolated dimer against the oracle;
 * a list index out of range is rejected (and the list dropped);
+* the host-array hooks' staged copies round-trip a 400,003-atom system;
 * the 2,048,000-atom four-species cascade lattice (fcc 80^3, 10 keV PKA, the
   benchmark's system), too large for the oracle in a test, through
   size-independent properties of one post_force / end_of_step: momentum of
@@ -191,6 +192,42 @@ def test_bad_neighbour_index_is_rejected(tmp_path):
     gpu.set_atoms(s)
     gpu.post_force(s, step=1)
     assert np.array_equal(gpu.get("f_EPH"), f_ref)
+
+
+def test_host_array_copies_round_trip_large(tmp_path):
+    """the host-array hooks' staged copies (pinned chunks of 8 MiB, split over
+    worker threads) on 400,003 atoms: several chunks and a short last one.
+    With f = 0 the integrator leaves v as it is and moves x by dt v, so the
+    arrays that come back are known bit for bit."""
+    n = 400_003
+    L = 200.0
+    rng = np.random.default_rng(11)
+    s = types.SimpleNamespace()
+    s.nlocal, s.nghost = n, 0
+    s.x = rng.uniform(0.0, L, (n, 3))
+    s.v = rng.normal(0.0, 5.0, (n, 3))
+    s.f = np.zeros((n, 3))
+    s.type = np.ones(n, np.int32); s.mask = np.ones(n, np.int32)
+    s.tag = np.arange(1, n + 1, dtype=np.int32)
+    s.ghost_owner = np.zeros(0, np.int32); s.ghost_shift = np.zeros((0, 3))
+    s.nbr_off = np.zeros(n + 1, np.int32); s.nbr_idx = np.zeros(0, np.int32)
+    p = str(tmp_path / "h.beta")
+    names = inputs.workload_beta(p, 1)
+    dt = 1e-3
+    fix = ea.FixEPH(inputs.fix_args(p, names, flags="3", grid=(4, 4, 4)), 1, [0, 0, 0], [L] * 3,
+                    dt, inputs.BOLTZ, inputs.FTM2V, masses=[inputs.MASS_CU])
+    fix.init()
+    fix.set_atoms(s)
+    x0, v0 = s.x.copy(), s.v.copy()
+    fix.initial_integrate(s)
+    assert np.array_equal(s.v, v0)
+    assert np.array_equal(s.x, x0 + dt * v0)
+    fix.post_force(s, step=1)  # no neighbours: f comes back as it went up
+    assert np.array_equal(s.f, np.zeros((n, 3)))
+    s.v[:] = rng.normal(0.0, 5.0, (n, 3))
+    v1 = s.v.copy()
+    fix.final_integrate(s)
+    assert np.array_equal(s.v, v1)
 
 
 def test_nonfinite_grid_is_reported(tmp_path):
