@@ -11,6 +11,7 @@ import glob
 import hashlib
 import os
 import shutil
+import subprocess
 
 import numpy as np
 import pytest
@@ -256,3 +257,20 @@ def test_product_kappa_tables_match_reference_goldens(tmp_path, name):
         rows, dx = ea.kappa_table(path, index, which)
         assert dx == float(d[key + "_dx"]), key
         assert sha(rows) == str(d[key + "_sha256"]), key
+
+
+def test_lammps_adaptor_compiles_against_the_lammps_api():
+    """user-eph_amd/lammps/fix_eph_amd.cpp against the LAMMPS-shaped headers of
+    tests/lammps_mock (the Fix API as the reference's fix_eph.cpp uses it);
+    the harness driver too"""
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    root = ROOT
+    mock = os.path.join(ROOT, "tests", "lammps_mock")
+    for src in (os.path.join(root, "user-eph_amd", "lammps", "fix_eph_amd.cpp"),
+                os.path.join(mock, "mock_driver.cpp")):
+        r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
+                            "-I" + mock, "-I" + os.path.join(root, "include"),
+                            "-I" + os.path.join(root, "user-eph_amd", "lammps"), src],
+                           capture_output=True, text=True)
+        assert r.returncode == 0, r.stderr

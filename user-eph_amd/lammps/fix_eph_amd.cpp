@@ -1,0 +1,196 @@
+/* fix_eph_amd.cpp -- LAMMPS adaptor over the C ABI of eph_amd.h (see
+ * fix_eph_amd.h).  Each hook forwards to the context with LAMMPS's own arrays:
+ * the context copies x, v, f in and out of HBM and runs the kernels; the
+ * atoms, the ghost map and the neighbour list are re-sent after every
+ * neighbour-list build, when LAMMPS may have exchanged, sorted or re-imaged
+ * atoms.  One MPI rank per GPU; the per-step ghost exchange of rho, w and xi
+ * (fix_eph.cpp:612, 744, 864, 871) happens inside the context, which holds
+ * every ghost's owner, so this adaptor serves single-rank runs. */
+#include "fix_eph_amd.h"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "atom.h"
+#include "comm.h"
+#include "domain.h"
+#include "error.h"
+#include "force.h"
+#include "memory.h"
+#include "neigh_list.h"
+#include "neigh_request.h"
+#include "neighbor.h"
+#include "update.h"
+
+extern "C" {
+#include "eph_amd.h"
+}
+
+using namespace LAMMPS_NS;
+using namespace FixConst;
+
+static_assert(sizeof(tagint) == sizeof(int), "fix eph/amd passes atom tags as int (no -DLAMMPS_BIGBIG)");
+
+namespace {
+// style name of the context: the reference name behind "<name>/amd"
+std::string reference_style(const char *s) {
+  std::string n(s);
+  const std::string tail = "/amd";
+  if (n.size() > tail.size() && n.compare(n.size() - tail.size(), tail.size(), tail) == 0)
+    n.erase(n.size() - tail.size());
+  return n;
+}
+}  // namespace
+
+FixEPHAMD::FixEPHAMD(LAMMPS *lmp, int narg, char **arg) :
+    Fix(lmp, narg, arg), ctx(nullptr), list(nullptr), ncols(8), array(nullptr), list_step(-1)
+{
+  if (narg < 3) error->all(FLERR, "Illegal fix eph/amd command");
+  if (comm->nprocs > 1) error->all(FLERR, "fix eph/amd runs on one MPI rank");
+  if (atom->map_style == Atom::MAP_NONE)
+    error->all(FLERR, "fix eph/amd needs an atom map (atom_modify map array)");
+  // the context parses the same words as fix eph, with the reference style
+  const std::string style = reference_style(arg[2]);
+  if (style != "eph" && style != "eph/coloured")
+    error->all(FLERR, "fix eph/amd: the style keeps per-atom state this adaptor does not migrate");
+  std::vector<const char *> words(arg, arg + narg);
+  words[2] = style.c_str();
+  const int rc = eph_create(&ctx, narg, words.data(), atom->ntypes, domain->boxlo, domain->boxhi,
+                            update->dt, force->boltz, force->ftm2v, /*device=*/0);
+  if (rc != EPH_OK) error->all(FLERR, eph_last_error(nullptr));
+  check(eph_set_groupbit(ctx, groupbit));
+  check(eph_set_masses(ctx, atom->mass, atom->ntypes));
+
+  vector_flag = 1;  // Ee and the mean T_e (fix_eph.cpp:406-428)
+  size_vector = 2;
+  global_freq = 1;
+  extvector = 1;
+  ncols = eph_peratom_cols(ctx);  // 8, or 12 for eph/atomic
+  peratom_flag = 1;
+  size_peratom_cols = ncols;
+  peratom_freq = 1;
+  comm->ghost_velocity = 1;  // fix_eph.cpp:82
+
+  grow_arrays(atom->nmax);
+  atom->add_callback(Atom::GROW);
+}
+
+FixEPHAMD::~FixEPHAMD()
+{
+  atom->delete_callback(id, Atom::GROW);
+  eph_destroy(ctx);
+  memory->destroy(array);
+}
+
+int FixEPHAMD::setmask()
+{
+  // eph_setmask(): all four hooks of fix_eph.cpp:293-302
+  return INITIAL_INTEGRATE | POST_FORCE | FINAL_INTEGRATE | END_OF_STEP;
+}
+
+void FixEPHAMD::init()
+{
+  double cutoff = 0.0;
+  const int rc = eph_init(ctx, domain->dimension, domain->nonperiodic, domain->triclinic, &cutoff);
+  if (rc != EPH_OK) error->all(FLERR, eph_last_error(ctx));
+  // a full list with ghosts at the rho(r) cutoff (fix_eph.cpp:273-275)
+  auto req = neighbor->add_request(this, NeighConst::REQ_FULL | NeighConst::REQ_GHOST);
+  req->set_cutoff(cutoff);
+}
+
+void FixEPHAMD::init_list(int, NeighList *ptr)
+{
+  list = ptr;
+}
+
+void FixEPHAMD::reset_dt()
+{
+  check(eph_reset_dt(ctx, update->dt));
+}
+
+void FixEPHAMD::initial_integrate(int)
+{
+  if (list_step < 0) upload_atoms();  // the first step runs before any post_force
+  check(eph_initial_integrate(ctx, &atom->x[0][0], &atom->v[0][0], &atom->f[0][0]));
+}
+
+void FixEPHAMD::post_force(int)
+{
+  if (neighbor->lastcall != list_step) {  // a list build: atoms may have moved in memory
+    upload_atoms();
+    upload_list();
+    list_step = neighbor->lastcall;
+  }
+  check(eph_set_timestep(ctx, update->ntimestep));
+  check(eph_post_force(ctx, &atom->x[0][0], &atom->v[0][0], &atom->f[0][0]));
+}
+
+void FixEPHAMD::final_integrate()
+{
+  check(eph_final_integrate(ctx, &atom->v[0][0], &atom->f[0][0]));
+}
+
+void FixEPHAMD::end_of_step()
+{
+  check(eph_end_of_step(ctx, &atom->x[0][0], &atom->v[0][0], update->ntimestep));
+  if (atom->nlocal > 0) check(eph_get_peratom(ctx, &array[0][0]));
+}
+
+double FixEPHAMD::compute_vector(int i)
+{
+  double v = 0.0;
+  check(eph_compute_vector(ctx, i, &v));
+  return v;
+}
+
+void FixEPHAMD::post_run()
+{
+  check(eph_post_run(ctx));
+}
+
+void FixEPHAMD::grow_arrays(int nmax)
+{
+  memory->grow(array, nmax, ncols, "eph/amd:array");
+  array_atom = array;
+}
+
+void FixEPHAMD::check(int rc)
+{
+  if (rc != EPH_OK) error->one(FLERR, eph_last_error(ctx));
+}
+
+// atoms and the ghost -> owner map; on one rank every ghost is a periodic
+// image of a local atom, found through atom->map of its tag.  The hooks send
+// the ghosts' own x and v every step, so no image offsets are needed.
+void FixEPHAMD::upload_atoms()
+{
+  const int nlocal = atom->nlocal, nghost = atom->nghost;
+  check(eph_set_atoms(ctx, nlocal, nghost, &atom->x[0][0], &atom->v[0][0], atom->type,
+                      atom->mask, atom->tag));
+  owner.resize(nghost);
+  for (int g = 0; g < nghost; ++g) {
+    const int o = atom->map(atom->tag[nlocal + g]);
+    if (o < 0 || o >= nlocal) error->one(FLERR, "fix eph/amd: ghost atom without a local owner");
+    owner[g] = o;
+  }
+  check(eph_set_ghost_map(ctx, nghost, owner.data(), nullptr));
+}
+
+// NeighList -> CSR (what transfer_neighbour_list of fix_eph_gpu.cpp:241-280 did)
+void FixEPHAMD::upload_list()
+{
+  const int n = list->inum;
+  off.assign(static_cast<size_t>(atom->nlocal) + 1, 0);
+  for (int ii = 0; ii < n; ++ii) {
+    const int i = list->ilist[ii];
+    off[i + 1] = list->numneigh[i];
+  }
+  for (int i = 0; i < atom->nlocal; ++i) off[i + 1] += off[i];
+  idx.resize(off.back());
+  for (int ii = 0; ii < n; ++ii) {
+    const int i = list->ilist[ii];
+    std::copy(list->firstneigh[i], list->firstneigh[i] + list->numneigh[i], idx.begin() + off[i]);
+  }
+  check(eph_set_neighbors(ctx, atom->nlocal, off.data(), idx.data(), NEIGHMASK));
+}
