@@ -41,6 +41,8 @@ class Fix : protected Pointers {
   virtual double compute_vector(int) { return 0.0; }
   virtual void post_run() {}
   virtual void grow_arrays(int) {}
+  virtual void pre_exchange() {}
+  virtual void copy_arrays(int, int, int) {}
 
  private:
   std::string id_store;

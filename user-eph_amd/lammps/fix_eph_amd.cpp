@@ -3,9 +3,12 @@
  * the context copies x, v, f in and out of HBM and runs the kernels; the
  * atoms, the ghost map and the neighbour list are re-sent after every
  * neighbour-list build, when LAMMPS may have exchanged, sorted or re-imaged
- * atoms.  One MPI rank per GPU; the per-step ghost exchange of rho, w and xi
- * (fix_eph.cpp:612, 744, 864, 871) happens inside the context, which holds
- * every ghost's owner, so this adaptor serves single-rank runs. */
+ * atoms.  Per-atom state of the stateful styles comes to LAMMPS's rows before
+ * the exchange (pre_exchange), moves with the atoms (copy_arrays) and goes
+ * back after the list build.  One MPI rank per GPU; the per-step ghost
+ * exchange of rho, w and xi (fix_eph.cpp:612, 744, 864, 871) happens inside
+ * the context, which holds every ghost's owner, so this adaptor serves
+ * single-rank runs. */
 #include "fix_eph_amd.h"
 
 #include <algorithm>
@@ -44,7 +47,8 @@ std::string reference_style(const char *s) {
 }  // namespace
 
 FixEPHAMD::FixEPHAMD(LAMMPS *lmp, int narg, char **arg) :
-    Fix(lmp, narg, arg), ctx(nullptr), list(nullptr), ncols(8), array(nullptr), list_step(-1)
+    Fix(lmp, narg, arg), ctx(nullptr), list(nullptr), ncols(8), array(nullptr), list_step(-1),
+    nstate(0), state(nullptr), state_on_host(false)
 {
   if (narg < 3) error->all(FLERR, "Illegal fix eph/amd command");
   if (comm->nprocs > 1) error->all(FLERR, "fix eph/amd runs on one MPI rank");
@@ -52,8 +56,12 @@ FixEPHAMD::FixEPHAMD(LAMMPS *lmp, int narg, char **arg) :
     error->all(FLERR, "fix eph/amd needs an atom map (atom_modify map array)");
   // the context parses the same words as fix eph, with the reference style
   const std::string style = reference_style(arg[2]);
-  if (style != "eph" && style != "eph/coloured")
-    error->all(FLERR, "fix eph/amd: the style keeps per-atom state this adaptor does not migrate");
+  if (style == "eph/coloured/exp" || style == "eph/coloured/exp/v1")
+    nstate = 6;
+  else if (style == "eph/atomic")
+    nstate = 1;
+  else if (style != "eph" && style != "eph/coloured")
+    error->all(FLERR, "fix eph/amd: unknown style " + style);
   std::vector<const char *> words(arg, arg + narg);
   words[2] = style.c_str();
   const int rc = eph_create(&ctx, narg, words.data(), atom->ntypes, domain->boxlo, domain->boxhi,
@@ -81,12 +89,16 @@ FixEPHAMD::~FixEPHAMD()
   atom->delete_callback(id, Atom::GROW);
   eph_destroy(ctx);
   memory->destroy(array);
+  memory->destroy(state);
 }
 
 int FixEPHAMD::setmask()
 {
-  // eph_setmask(): all four hooks of fix_eph.cpp:293-302
-  return INITIAL_INTEGRATE | POST_FORCE | FINAL_INTEGRATE | END_OF_STEP;
+  // eph_setmask(): all four hooks of fix_eph.cpp:293-302, and the state
+  // hand-over before atoms move
+  int mask = INITIAL_INTEGRATE | POST_FORCE | FINAL_INTEGRATE | END_OF_STEP;
+  if (nstate > 0) mask |= PRE_EXCHANGE;
+  return mask;
 }
 
 void FixEPHAMD::init()
@@ -119,6 +131,7 @@ void FixEPHAMD::post_force(int)
 {
   if (neighbor->lastcall != list_step) {  // a list build: atoms may have moved in memory
     upload_atoms();
+    if (state_on_host) state_to_device();
     upload_list();
     list_step = neighbor->lastcall;
   }
@@ -153,6 +166,55 @@ void FixEPHAMD::grow_arrays(int nmax)
 {
   memory->grow(array, nmax, ncols, "eph/amd:array");
   array_atom = array;
+  if (nstate > 0) memory->grow(state, nmax, nstate, "eph/amd:state");
+}
+
+// before atoms are exchanged or sorted: the context's per-atom state into the
+// rows that travel with the atoms
+void FixEPHAMD::pre_exchange()
+{
+  if (nstate > 0 && list_step >= 0) state_to_host();
+}
+
+void FixEPHAMD::copy_arrays(int i, int j, int)
+{
+  for (int k = 0; k < nstate; ++k) state[j][k] = state[i][k];
+}
+
+void FixEPHAMD::state_to_host()
+{
+  const int n = atom->nlocal;
+  if (nstate == 6) {
+    buf.resize(3 * static_cast<size_t>(n));
+    for (int which = 0; which < 2; ++which) {
+      if (n) check(eph_get_filter_state(ctx, which, buf.data()));
+      for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) state[i][3 * which + d] = buf[3 * i + d];
+    }
+  } else if (n > 0) {
+    buf.resize(n);
+    check(eph_get_atomic_energy(ctx, buf.data()));
+    for (int i = 0; i < n; ++i) state[i][0] = buf[i];
+  }
+  state_on_host = true;
+}
+
+void FixEPHAMD::state_to_device()
+{
+  const int n = atom->nlocal;
+  if (nstate == 6) {
+    buf.resize(3 * static_cast<size_t>(n));
+    for (int which = 0; which < 2; ++which) {
+      for (int i = 0; i < n; ++i)
+        for (int d = 0; d < 3; ++d) buf[3 * i + d] = state[i][3 * which + d];
+      if (n) check(eph_set_filter_state(ctx, which, buf.data()));
+    }
+  } else if (n > 0) {
+    buf.resize(n);
+    for (int i = 0; i < n; ++i) buf[i] = state[i][0];
+    check(eph_set_atomic_energy(ctx, buf.data()));
+  }
+  state_on_host = false;
 }
 
 void FixEPHAMD::check(int rc)

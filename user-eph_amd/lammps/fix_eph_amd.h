@@ -1,17 +1,21 @@
 /* fix_eph_amd.h -- LAMMPS adaptor: `fix ID group eph/amd ...` runs the fix eph
  * hot path on an MI355X through the C ABI of eph_amd.h.  Copy this file and
  * fix_eph_amd.cpp into a LAMMPS src/ tree (with include/eph_amd.h on the
- * include path and libeph_amd.so on the link line).  The command words after
- * the style name are those of `fix eph` (fix_eph.cpp:61-241).  The style name
- * without "/amd" is the reference style the context runs: eph and
- * eph/coloured.  The exp-filtered coloured styles and eph/atomic keep per-atom
- * state from step to step; carrying it with migrating atoms needs the
- * grow/copy/pack_exchange callbacks (eph_get/set_filter_state,
- * eph_get/set_atomic_energy), which this adaptor does not implement. */
+ * include path and libeph_amd.so on the link line).  The style name without
+ * "/amd" is the reference style the context runs, with that style's command
+ * words: eph, eph/coloured, eph/coloured/exp, eph/coloured/exp/v1
+ * (fix_eph.cpp:61-241, fix_eph_coloured_exp.cpp:43) and eph/atomic
+ * (fix_eph_atomic.cpp:59-269).  The exp-filtered coloured styles and
+ * eph/atomic keep per-atom state from step to step (filter states, electronic
+ * energies); the adaptor holds it in LAMMPS per-atom rows while atoms are
+ * sorted or exchanged, as the reference fixes' grow/copy callbacks do. */
 #ifdef FIX_CLASS
 // clang-format off
 FixStyle(eph/amd,FixEPHAMD)
 FixStyle(eph/coloured/amd,FixEPHAMD)
+FixStyle(eph/coloured/exp/amd,FixEPHAMD)
+FixStyle(eph/coloured/exp/v1/amd,FixEPHAMD)
+FixStyle(eph/atomic/amd,FixEPHAMD)
 // clang-format on
 #else
 
@@ -41,6 +45,8 @@ class FixEPHAMD : public Fix {
   double compute_vector(int) override;
   void post_run() override;
   void grow_arrays(int) override;
+  void pre_exchange() override;
+  void copy_arrays(int, int, int) override;
 
  private:
   eph_context *ctx;
@@ -49,6 +55,14 @@ class FixEPHAMD : public Fix {
   double **array;
   bigint list_step;  // neighbor->lastcall of the list on the device, -1 none
   std::vector<int> off, idx, owner;
+  // per-atom state carried by LAMMPS while atoms move: 6 columns (the two
+  // filter states of eph/coloured/exp*), 1 (E_i of eph/atomic) or none
+  int nstate;
+  double **state;
+  bool state_on_host;  // the rows hold the context's state (since pre_exchange)
+  std::vector<double> buf;
+  void state_to_host();
+  void state_to_device();
 
   void check(int rc);
   void upload_atoms();
