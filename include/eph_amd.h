@@ -184,12 +184,16 @@ void* eph_stream(eph_context* ctx);
 
 /* Kernel forms of the two neighbour sweeps (no counterpart in the reference;
  * every form gives the same bits).  0: one 256-thread block per 256 atoms,
- * spline tables read through the caches; 1: persistent 1024-thread blocks
- * with the rho(r^2) tables staged in LDS (only when they fit); -1: chosen by
- * system size (the default).  eph_autotune times both forms of each sweep on
+ * spline tables read through the caches; 1, 2, 3: persistent blocks of 1024,
+ * 512 or 256 threads (one per CU: 4, 2 or 1 waves per SIMD) with the rho(r^2)
+ * tables staged in LDS (only when they fit); -1: chosen by system size (the
+ * default: LDS forms for large systems, 1024 threads for the density sweep and
+ * 512 for the force sweep).  eph_autotune times every form of each sweep on
  * the resident atoms (after eph_dev_post_force or the harness's attach) and
- * keeps the faster; forms[2] receives the choice (density, force) and ms[4]
- * the times (density form 0, 1, force form 0, 1).  f is not changed. */
+ * keeps the fastest; forms[2] receives the choice (density, force) and
+ * ms[2 * EPH_SWEEP_FORMS] the times (density forms 0..3, then force forms
+ * 0..3; 0 for a form the device cannot run).  f is not changed. */
+#define EPH_SWEEP_FORMS 4
 int eph_set_sweep_forms(eph_context* ctx, int density, int force);
 int eph_autotune(eph_context* ctx, int* forms, double* ms);
 

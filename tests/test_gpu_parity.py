@@ -94,11 +94,12 @@ def test_alloy_cascade_small(oracle, tmp_path, forms):
 
 
 @pytest.mark.parametrize("forms", [("auto", "auto"), ("chunk", "chunk"), ("chunk", "lds"),
-                                   ("lds", "chunk")])
+                                   ("lds", "chunk"), ("lds512", "lds256"), ("lds256", "lds512")])
 def test_alloy_lds_table_path(oracle, tmp_path, forms):
     """fcc 21^3 (37,044 atoms), 4 species: large enough that the default is the
     persistent kernels staging all four rho(r^2) tables (128 KB) in LDS; every
-    combination of sweep forms gives the oracle's bits."""
+    combination of sweep forms (and LDS block sizes 1024, 512, 256) gives the
+    oracle's bits."""
     p = Pair(oracle, ea, str(tmp_path), cells=21, species=4, T=900.0, disp=0.05, grid=7, dt=5e-4)
     p.gpu.set_sweep_forms(*forms)
     for _ in range(3):
@@ -115,7 +116,9 @@ def test_autotune_keeps_forces_and_bits(oracle, tmp_path):
     after = p.gpu.dev_atoms()
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
-    assert tuned["density"] in ("chunk", "lds") and tuned["force"] in ("chunk", "lds")
+    forms = ("chunk", "lds", "lds512", "lds256")
+    assert tuned["density"] in forms and tuned["force"] in forms
+    assert set(tuned["density_ms"]) == set(forms) == set(tuned["force_ms"]), tuned
     assert all(v > 0 for v in tuned["density_ms"].values()), tuned
     for _ in range(2):
         p.step()

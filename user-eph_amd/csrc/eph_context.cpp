@@ -464,7 +464,7 @@ int eph_context::dev_post_force() {
   return EPH_OK;
 }
 
-// Times both forms of each sweep on the resident atoms and keeps the faster.
+// Times every form of each sweep on the resident atoms and keeps the fastest.
 // The density sweep rewrites only what the next post_force rewrites anyway;
 // the force sweep runs with both force additions off, so f is not touched.
 int eph_context::autotune(int* forms, double* ms) {
@@ -482,10 +482,10 @@ int eph_context::autotune(int* forms, double* ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t he = hipEventCreate(&e0);
   if (he == hipSuccess) he = hipEventCreate(&e1);
-  double t[4] = {0, 0, 0, 0};
-  const int nv = lds_tables_possible(tabs) ? 2 : 1;
+  constexpr int nf = kSweepForms;
+  double t[2 * nf] = {};
+  const int nv = lds_tables_possible(tabs) ? nf : 1;  // forms 1.. need the tables in LDS
   for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
-    if (phase == 1 && model != 4) break;
     for (int v = 0; v < nv && he == hipSuccess; ++v) {
       for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // the first launch warms up
         he = hipEventRecord(e0, stream);
@@ -496,17 +496,23 @@ int eph_context::autotune(int* forms, double* ms) {
         if (he == hipSuccess) he = hipEventSynchronize(e1);
         float m = 0;
         if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
-        if (rep > 0) t[2 * phase + v] += m / 3.0;
+        if (rep > 0) t[nf * phase + v] += m / 3.0;
       }
     }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (he != hipSuccess) return fail_hip(he, "autotune");
-  form_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
-  form_force = (nv > 1 && model == 4 && t[3] < t[2]) ? 1 : 0;
+  auto best = [&](const double* tt) {
+    int b = 0;
+    for (int v = 1; v < nv; ++v)
+      if (tt[v] < tt[b]) b = v;
+    return b;
+  };
+  form_density = best(t);
+  form_force = best(t + nf);
   if (forms) { forms[0] = form_density; forms[1] = form_force; }
-  if (ms) std::copy(t, t + 4, ms);
+  if (ms) std::copy(t, t + 2 * nf, ms);
   return EPH_OK;
 }
 
@@ -1124,8 +1130,9 @@ int eph_sync(eph_context* ctx) {
 void* eph_stream(eph_context* ctx) { return ctx ? (void*)ctx->stream : nullptr; }
 
 int eph_set_sweep_forms(eph_context* ctx, int density, int force) {
-  if (!ctx || density < -1 || density > 1 || force < -1 || force > 1) return EPH_ERR_ARGS;
-  if ((density == 1 || force == 1) && !lds_tables_possible(ctx->tabs))
+  if (!ctx || density < -1 || density >= kSweepForms || force < -1 || force >= kSweepForms)
+    return EPH_ERR_ARGS;
+  if ((density >= 1 || force >= 1) && !lds_tables_possible(ctx->tabs))
     return ctx->fail(EPH_ERR_ARGS, "the spline tables do not fit in LDS on this device");
   ctx->form_density = density;
   ctx->form_force = force;

@@ -132,8 +132,12 @@ struct DepositView {
 constexpr int kBlock = 256;
 constexpr int kCellConstant = 0, kCellDynamic = 1, kCellZeroDerivative = 2;
 constexpr int kMaxSortInCell = 64;
-// block size of the persistent LDS-table forms of the neighbour sweeps
+// block size of the persistent LDS-table forms of the ensemble's sweeps
 constexpr int kBigBlock = 1024;
+// kernel forms of the two neighbour sweeps: 0 reads the rho(r^2) tables
+// through the caches; the others stage them in LDS, with 1024, 512 or 256
+// threads per block (one block per CU)
+constexpr int kLdsForm1024 = 1, kLdsForm512 = 2, kLdsForm256 = 3, kSweepForms = 4;
 // rho(r^2) tables up to this size are staged in LDS (160 KB per CU on gfx950)
 constexpr size_t kLdsTableMax = 128 * 1024;
 constexpr int kLdsMinAtoms = 32768;
@@ -148,8 +152,10 @@ hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* 
 // indices outside [0, nall) set *bad (and are stored as 0)
 hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int nall,
                              int* ell, int* nn, int stride, int* bad, hipStream_t s);
-// form: 0 one block per 256 atoms with tables read from global memory,
-// 1 persistent blocks with the tables in LDS (when they fit), -1 by size
+// form: 0 one block per 256 atoms with tables read from global memory;
+// kLdsForm1024/512/256 persistent blocks of that many threads with the
+// tables in LDS (when they fit); -1 by size (LDS with 1024 threads for the
+// density sweep, 512 for the force sweep)
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
                             const WorkView& w, hipStream_t s, int form = -1);
 // the tables fit in one workgroup's LDS on the current device

@@ -215,19 +215,21 @@ __device__ __forceinline__ int xcd_tiles_end(int ntiles) {
   return static_cast<int>((static_cast<long>(ntiles) * (x + 1)) / kXcd);
 }
 
-template <int XI, bool FRIC, bool MULTI>
-__global__ __launch_bounds__(kBigBlock) void k_density_w_lds(AtomsView a, TablesView t,
-                                                             StepParams p, WorkView w) {
+// BLK threads per block: one block per CU either way (the tables take most
+// of the LDS), so BLK sets the waves per SIMD (1024: 4, 512: 2, 256: 1).
+template <int XI, bool FRIC, bool MULTI, int BLK>
+__global__ __launch_bounds__(BLK) void k_density_w_lds(AtomsView a, TablesView t,
+                                                       StepParams p, WorkView w) {
   extern __shared__ double4 s_tab[];
   const int ntab = t.n_elem * t.n_rho;
-  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  for (int k = threadIdx.x; k < ntab; k += BLK) s_tab[k] = t.rho_r2[k];
   __syncthreads();
-  const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+  const int ntiles = (a.nlocal + BLK - 1) / BLK;
   int tile, step;
   xcd_tiles(ntiles, &tile, &step);
   const int end = xcd_tiles_end(ntiles);
   for (; tile < end; tile += step) {
-    const int i = tile * kBigBlock + threadIdx.x;
+    const int i = tile * BLK + threadIdx.x;
     if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI, false>(i, a, t, s_tab, p, w);
   }
 }
@@ -381,19 +383,19 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t
   if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, t.rho_r2, g, p, w);
 }
 
-template <bool FRIC, bool RAND, bool MULTI>
-__global__ __launch_bounds__(kBigBlock) void k_prl_forces_lds(AtomsView a, TablesView t, GridView g,
-                                                              StepParams p, WorkView w) {
+template <bool FRIC, bool RAND, bool MULTI, int BLK>
+__global__ __launch_bounds__(BLK) void k_prl_forces_lds(AtomsView a, TablesView t, GridView g,
+                                                        StepParams p, WorkView w) {
   extern __shared__ double4 s_tab[];
   const int ntab = t.n_elem * t.n_rho;
-  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  for (int k = threadIdx.x; k < ntab; k += BLK) s_tab[k] = t.rho_r2[k];
   __syncthreads();
-  const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
+  const int ntiles = (a.nlocal + BLK - 1) / BLK;
   int tile, step;
   xcd_tiles(ntiles, &tile, &step);
   const int end = xcd_tiles_end(ntiles);
   for (; tile < end; tile += step) {
-    const int i = tile * kBigBlock + threadIdx.x;
+    const int i = tile * BLK + threadIdx.x;
     if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, s_tab, g, p, w);
   }
 }
@@ -1321,11 +1323,11 @@ hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int nei
 // Grid of a persistent LDS-table launch: all blocks that fit at once, a
 // multiple of the 8 XCDs, no more than the tiles need.
 template <typename K>
-static int lds_grid(K kernel, size_t lds_bytes, int ntiles) {
+static int lds_grid(K kernel, size_t lds_bytes, int ntiles, int block = kBigBlock) {
   (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                             hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds_bytes));
   int occ = 0, dev = 0, cus = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, kBigBlock, lds_bytes) != hipSuccess ||
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, kernel, block, lds_bytes) != hipSuccess ||
       occ < 1)
     occ = 1;
   (void)hipGetDevice(&dev);
@@ -1357,18 +1359,40 @@ static bool use_lds_tables(const AtomsView& a, const TablesView& t) {
 
 bool lds_tables_possible(const TablesView& t) { return lds_tables_fit(t); }
 
+// The LDS-table form of a sweep (kLdsForm1024 .. kLdsForm256), or 0 for the
+// form that reads the tables through the caches.  -1 picks by system size,
+// with `dflt` for large systems.
+static int resolve_form(const AtomsView& a, const TablesView& t, int form, int dflt) {
+  if (form == 0 || !lds_tables_fit(t)) return 0;
+  if (form < 0) return use_lds_tables(a, t) ? dflt : 0;
+  return form;
+}
+
+template <int XI, bool F, bool M>
+static void density_lds(int blk, size_t lds, const AtomsView& a, const TablesView& t,
+                        const StepParams& p, const WorkView& w, hipStream_t s) {
+#define EPH_DWL(B)                                                                               \
+  k_density_w_lds<XI, F, M, B><<<lds_grid(k_density_w_lds<XI, F, M, B>, lds,                    \
+                                          (a.nlocal + B - 1) / B, B), B, lds, s>>>(a, t, p, w)
+  if (blk == 256) EPH_DWL(256);
+  else if (blk == 512) EPH_DWL(512);
+  else EPH_DWL(1024);
+#undef EPH_DWL
+}
+
+static int lds_block(int form) { return form == kLdsForm256 ? 256 : form == kLdsForm512 ? 512 : 1024; }
+
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
                             const WorkView& w, hipStream_t s, int form) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
-  if (form != 0 && lds_tables_fit(t) && (form == 1 || use_lds_tables(a, t))) {
+  const int f = resolve_form(a, t, form, kLdsForm1024);
+  if (f != 0) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-    const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
-#define EPH_DWL(XI, F, M)                                                                  \
-  k_density_w_lds<XI, F, M><<<lds_grid(k_density_w_lds<XI, F, M>, lds, ntiles), kBigBlock, lds, \
-                              s>>>(a, t, p, w)
-#define EPH_DWL_M(XI, F) if (multi) EPH_DWL(XI, F, true); else EPH_DWL(XI, F, false)
+    const int blk = lds_block(f);
+#define EPH_DWL_M(XI, F) if (multi) density_lds<XI, F, true>(blk, lds, a, t, p, w, s); \
+                         else density_lds<XI, F, false>(blk, lds, a, t, p, w, s)
 #define EPH_DWL_F(XI) if (fric) { EPH_DWL_M(XI, true); } else { EPH_DWL_M(XI, false); }
     switch (p.xi_mode) {
       case 1: EPH_DWL_F(1); break;
@@ -1377,7 +1401,6 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
     }
 #undef EPH_DWL_F
 #undef EPH_DWL_M
-#undef EPH_DWL
     return hipGetLastError();
   }
   const int g = nblk(a.nlocal);
@@ -1408,23 +1431,33 @@ hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s
   return hipGetLastError();
 }
 
+template <bool F, bool R, bool M>
+static void forces_lds(int blk, size_t lds, const AtomsView& a, const TablesView& t,
+                       const GridView& g, const StepParams& p, const WorkView& w, hipStream_t s) {
+#define EPH_PFL(B)                                                                              \
+  k_prl_forces_lds<F, R, M, B><<<lds_grid(k_prl_forces_lds<F, R, M, B>, lds,                   \
+                                          (a.nlocal + B - 1) / B, B), B, lds, s>>>(a, t, g, p, w)
+  if (blk == 256) EPH_PFL(256);
+  else if (blk == 512) EPH_PFL(512);
+  else EPH_PFL(1024);
+#undef EPH_PFL
+}
+
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
                              const StepParams& p, const WorkView& w, hipStream_t s, int form) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (!fric && !rnd) return hipSuccess;
-  if (form != 0 && lds_tables_fit(t) && (form == 1 || use_lds_tables(a, t))) {
+  const int f = resolve_form(a, t, form, kLdsForm512);
+  if (f != 0) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-    const int ntiles = (a.nlocal + kBigBlock - 1) / kBigBlock;
-#define EPH_PFL(F, R, M)                                                                 \
-  k_prl_forces_lds<F, R, M><<<lds_grid(k_prl_forces_lds<F, R, M>, lds, ntiles), kBigBlock, lds, \
-                              s>>>(a, t, g, p, w)
-#define EPH_PFL_M(F, R) if (multi) EPH_PFL(F, R, true); else EPH_PFL(F, R, false)
+    const int blk = lds_block(f);
+#define EPH_PFL_M(F, R) if (multi) forces_lds<F, R, true>(blk, lds, a, t, g, p, w, s); \
+                        else forces_lds<F, R, false>(blk, lds, a, t, g, p, w, s)
     if (fric && rnd) { EPH_PFL_M(true, true); }
     else if (fric) { EPH_PFL_M(true, false); }
     else { EPH_PFL_M(false, true); }
 #undef EPH_PFL_M
-#undef EPH_PFL
     return hipGetLastError();
   }
   const int gr = nblk(a.nlocal);

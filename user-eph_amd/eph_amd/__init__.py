@@ -390,21 +390,25 @@ class FixEPH:
         a = np.ascontiguousarray(values, dtype=np.float64).reshape(self.nlocal, 3)
         self._check(lib().eph_set_filter_state(self.h, which, _d(a)))
 
-    _FORMS = {"chunk": 0, "lds": 1, "auto": -1}
+    # kernel forms of the sweeps: tables through the caches, or in LDS with
+    # 1024 / 512 / 256 threads per block
+    _FORM_NAMES = ("chunk", "lds", "lds512", "lds256")
+    _FORMS = {"chunk": 0, "lds": 1, "lds512": 2, "lds256": 3, "auto": -1}
 
     def set_sweep_forms(self, density="auto", force="auto"):
         """kernel forms of the density and force sweeps: 'chunk', 'lds' or 'auto'"""
         self._check(lib().eph_set_sweep_forms(self.h, self._FORMS[density], self._FORMS[force]))
 
     def autotune(self):
-        """time both forms of each sweep on the resident atoms, keep the faster"""
+        """time every form of each sweep on the resident atoms, keep the fastest"""
+        nf = len(self._FORM_NAMES)
         forms = (C.c_int * 2)()
-        ms = (C.c_double * 4)()
+        ms = (C.c_double * (2 * nf))()
         self._check(lib().eph_autotune(self.h, forms, ms))
-        name = {0: "chunk", 1: "lds"}
+        name = self._FORM_NAMES
         return {"density": name[forms[0]], "force": name[forms[1]],
-                "density_ms": {"chunk": ms[0], "lds": ms[1]},
-                "force_ms": {"chunk": ms[2], "lds": ms[3]}}
+                "density_ms": {name[v]: ms[v] for v in range(nf) if ms[v] > 0},
+                "force_ms": {name[v]: ms[nf + v] for v in range(nf) if ms[nf + v] > 0}}
 
     def timing_report(self):
         L = lib()
