@@ -184,11 +184,12 @@ void* eph_stream(eph_context* ctx);
 
 /* Kernel forms of the two neighbour sweeps (no counterpart in the reference;
  * every form gives the same bits).  0: one 256-thread block per 256 atoms,
- * spline tables read through the caches; 1, 2, 3: persistent blocks of 1024,
- * 512 or 256 threads (one per CU: 4, 2 or 1 waves per SIMD) with the rho(r^2)
- * tables staged in LDS (only when they fit); -1: chosen by system size (the
- * default: LDS forms for large systems, 1024 threads for the density sweep and
- * 512 for the force sweep).  eph_autotune times every form of each sweep on
+ * spline tables read through the caches (the force sweep's waves fetch their
+ * neighbours' records together, 8 lanes per record, through LDS); 1, 2, 3:
+ * persistent blocks of 1024, 512 or 256 threads (one per CU: 4, 2 or 1 waves
+ * per SIMD) with the rho(r^2) tables staged in LDS (only when they fit); -1:
+ * chosen by system size (the default: form 1 for the density sweep of large
+ * systems, form 0 otherwise).  eph_autotune times every form of each sweep on
  * the resident atoms (after eph_dev_post_force or the harness's attach) and
  * keeps the fastest; forms[2] receives the choice (density, force) and
  * ms[2 * EPH_SWEEP_FORMS] the times (density forms 0..3, then force forms

@@ -263,73 +263,54 @@ __global__ __launch_bounds__(kBlock) void k_pack_pv(AtomsView a, WorkView w, int
 // fix_eph.cpp:748-787 (friction) and :791-836 (random), then the force
 // application of :892-906.  Both sums share the neighbour gather; each
 // accumulates in list order exactly as its reference loop does.
-// Each neighbour's record is loaded one iteration ahead of its use; with DEEP
-// its index is loaded two ahead, so the record load never waits for an index
-// load issued in the same iteration (more registers: the LDS-table forms,
-// whose occupancy is set by LDS, use it).
-template <bool FRIC, bool RAND, bool MULTI, bool DEEP = false>
-__device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a, const TablesView& t,
-                                                const double4* __restrict__ rtab,
-                                                const GridView& g, const StepParams& p,
-                                                const WorkView& w) {
-  const int stride = a.stride;
-  const bool grp = (a.mask[i] & a.groupbit) != 0;
-  const double4 rwi = w.rec[i].rw;
-  const double rho_i = rwi.w;
-  double fex = 0.0, fey = 0.0, fez = 0.0, frx = 0.0, fry = 0.0, frz = 0.0;
-  if (grp && rho_i > 0.0) {
-    const double4 pi = w.rec[i].p;
-    const double4 xai = w.rec[i].xa;
-    const double alpha_i = xai.w;
-    const int ei = MULTI ? static_cast<int>(pi.w) : 0;
-    const int na = w.nact[i];
-    int jn = na > 0 ? w.act[i] : 0;
-    double4 nrw = rwi, np = pi, nxa = xai;
-    if (na > 0) {
-      nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
-    }
-    if (DEEP) jn = na > 1 ? w.act[stride + i] : 0;  // index of entry k + 1
-    for (int k = 0; k < na; ++k) {
-      const double4 rwj = nrw, pj = np, xaj = nxa;
-      if (k + 1 < na) {
-        if (DEEP) {
-          const int jc = jn;
-          if (k + 2 < na) jn = w.act[(k + 2) * stride + i];
-          nrw = w.rec[jc].rw; np = w.rec[jc].p; nxa = w.rec[jc].xa;
-        } else {
-          jn = w.act[(k + 1) * stride + i];
-          nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
-        }
-      }
-      const double rho_j = rwj.w;
-      if (!(rho_j > 0.0)) continue;
-      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-      const double r2 = ex * ex + ey * ey + ez * ez;
-      const double alpha_j = xaj.w;
-      const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-      const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
-      const double vr_ij = MULTI ? cubic_at(rtab + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
-      if (FRIC) {
-        const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
-        const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
-        const double ev2 = ex * rwj.x + ey * rwj.y + ez * rwj.z;
-        const double var2 = alpha_j * vr_ij * ev2 / (rho_j * r2);
-        const double dvar = var1 - var2;
-        fex -= dvar * ex;
-        fey -= dvar * ey;
-        fez -= dvar * ez;
-      }
-      if (RAND) {
-        const double ex1 = ex * xai.x + ey * xai.y + ez * xai.z;
-        const double var1 = alpha_i * vr_ji * ex1 / (rho_i * r2);
-        const double ex2 = ex * xaj.x + ey * xaj.y + ez * xaj.z;
-        const double var2 = alpha_j * vr_ij * ex2 / (rho_j * r2);
-        const double dvar = var1 - var2;
-        frx += dvar * ex;
-        fry += dvar * ey;
-        frz += dvar * ez;
-      }
-    }
+
+// The terms of one neighbour j (rho_j > 0) of atom i, added to the friction
+// (fe) and random (fr) sums: one definition for every form of the sweep, so
+// that all of them round alike.
+template <bool FRIC, bool RAND, bool MULTI>
+__device__ __forceinline__ void prl_pair(const double4& pi, const double4& rwi, const double4& xai,
+                                         const int ei, const double4& pj, const double4& rwj,
+                                         const double4& xaj, const TablesView& t,
+                                         const double4* __restrict__ rtab, double* fe, double* fr) {
+  const double rho_i = rwi.w, alpha_i = xai.w;
+  const double rho_j = rwj.w, alpha_j = xaj.w;
+  const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+  const double r2 = ex * ex + ey * ey + ez * ez;
+  const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+  const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+  const double vr_ij = MULTI ? cubic_at(rtab + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
+  if (FRIC) {
+    const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
+    const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
+    const double ev2 = ex * rwj.x + ey * rwj.y + ez * rwj.z;
+    const double var2 = alpha_j * vr_ij * ev2 / (rho_j * r2);
+    const double dvar = var1 - var2;
+    fe[0] -= dvar * ex;
+    fe[1] -= dvar * ey;
+    fe[2] -= dvar * ez;
+  }
+  if (RAND) {
+    const double ex1 = ex * xai.x + ey * xai.y + ez * xai.z;
+    const double var1 = alpha_i * vr_ji * ex1 / (rho_i * r2);
+    const double ex2 = ex * xaj.x + ey * xaj.y + ez * xaj.z;
+    const double var2 = alpha_j * vr_ij * ex2 / (rho_j * r2);
+    const double dvar = var1 - var2;
+    fr[0] += dvar * ex;
+    fr[1] += dvar * ey;
+    fr[2] += dvar * ez;
+  }
+}
+
+// After the sums: the sqrt(T_e) factor of the random force, the coloured/exp
+// filters, the stored f_EPH / f_RNG and the force update.  `on` = atom i is in
+// the group with rho_i > 0 (otherwise its sums are zero and stay so).
+template <bool FRIC, bool RAND>
+__device__ __forceinline__ void prl_finish(const int i, const bool on, const double4& pi,
+                                           double* fe, double* fr, const AtomsView& a,
+                                           const GridView& g, const StepParams& p,
+                                           const WorkView& w) {
+  double fex = fe[0], fey = fe[1], fez = fe[2], frx = fr[0], fry = fr[1], frz = fr[2];
+  if (on) {
     if (RAND) {
       const double Te = g.T[cell_of(g, pi.x, pi.y, pi.z)];
       const double var = p.eta * sqrt(Te);
@@ -376,11 +357,152 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
   }
 }
 
+// One thread per atom, each gathering its own neighbours' records.  Each
+// record is loaded one iteration ahead of its use; with DEEP its index is
+// loaded two ahead, so the record load never waits for an index load issued
+// in the same iteration (more registers: the LDS-table forms, whose occupancy
+// is set by LDS, use it).
+template <bool FRIC, bool RAND, bool MULTI, bool DEEP = false>
+__device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a, const TablesView& t,
+                                                const double4* __restrict__ rtab,
+                                                const GridView& g, const StepParams& p,
+                                                const WorkView& w) {
+  const int stride = a.stride;
+  const bool grp = (a.mask[i] & a.groupbit) != 0;
+  const double4 rwi = w.rec[i].rw;
+  const double rho_i = rwi.w;
+  double fe[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
+  const bool on = grp && rho_i > 0.0;
+  double4 pi = rwi;
+  if (on) {
+    pi = w.rec[i].p;
+    const double4 xai = w.rec[i].xa;
+    const int ei = MULTI ? static_cast<int>(pi.w) : 0;
+    const int na = w.nact[i];
+    int jn = na > 0 ? w.act[i] : 0;
+    double4 nrw = rwi, np = pi, nxa = xai;
+    if (na > 0) {
+      nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+    }
+    if (DEEP) jn = na > 1 ? w.act[stride + i] : 0;  // index of entry k + 1
+    for (int k = 0; k < na; ++k) {
+      const double4 rwj = nrw, pj = np, xaj = nxa;
+      if (k + 1 < na) {
+        if (DEEP) {
+          const int jc = jn;
+          if (k + 2 < na) jn = w.act[(k + 2) * stride + i];
+          nrw = w.rec[jc].rw; np = w.rec[jc].p; nxa = w.rec[jc].xa;
+        } else {
+          jn = w.act[(k + 1) * stride + i];
+          nrw = w.rec[jn].rw; np = w.rec[jn].p; nxa = w.rec[jn].xa;
+        }
+      }
+      if (!(rwj.w > 0.0)) continue;
+      prl_pair<FRIC, RAND, MULTI>(pi, rwi, xai, ei, pj, rwj, xaj, t, rtab, fe, fr);
+    }
+  }
+  prl_finish<FRIC, RAND>(i, on, pi, fe, fr, a, g, p, w);
+}
+
+// Cooperative form: the wave fetches its 64 atoms' k-th neighbour records
+// together, 8 lanes per 128-B record, straight into LDS (global_load_lds, 16 B
+// per lane), and each lane then reads its own record from there.  A gather
+// instruction thus touches 8 cache lines instead of 64.  In copy m, lane
+// l = 8q + s fetches slot s of the record of lane 8m + q, which holds part
+// (s - q) mod 8 (parts 0-5: p, rw, xa; 6-7: padding, not fetched); the
+// rotation makes the per-lane reads free of bank conflicts.  Copy m lands at
+// stage + 72 m (16-B units: 1 KB plus 128 B of padding, so that lanes L and
+// L + 8 fall in different bank halves).  The records of entry k + 1 are in
+// flight while entry k is computed.  Every lane of the wave runs the loop to
+// the wave's longest active list (lanes past their own do only the copies),
+// so the whole wave must call this; `valid` = lane i is a local atom.
+// stage: 576 x 16 B of LDS owned by the calling wave.
+static __device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kCoopStage = 576;  // 16-B units of LDS per wave
+
+static __device__ __forceinline__ void coop_fetch(const double2* __restrict__ recs, int j, int q8,
+                                                  int part, double2* stage) {
+  // the record indices of lanes q8, 8 + q8, ..., 56 + q8, all read before the
+  // first copy is issued
+  int r0 = __shfl(j, q8), r1 = __shfl(j, 8 + q8), r2 = __shfl(j, 16 + q8), r3 = __shfl(j, 24 + q8);
+  int r4 = __shfl(j, 32 + q8), r5 = __shfl(j, 40 + q8), r6 = __shfl(j, 48 + q8), r7 = __shfl(j, 56 + q8);
+  asm volatile("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3), "+v"(r4), "+v"(r5), "+v"(r6), "+v"(r7));
+  if (part < 6) {
+    using lds_ptr = __attribute__((address_space(3))) void*;
+    const double2* g = recs + part;
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r0) * 8, (lds_ptr)(stage + 0), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r1) * 8, (lds_ptr)(stage + 72), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r2) * 8, (lds_ptr)(stage + 144), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r3) * 8, (lds_ptr)(stage + 216), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r4) * 8, (lds_ptr)(stage + 288), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r5) * 8, (lds_ptr)(stage + 360), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r6) * 8, (lds_ptr)(stage + 432), 16, 0, 0);
+    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r7) * 8, (lds_ptr)(stage + 504), 16, 0, 0);
+  }
+}
+
+template <bool FRIC, bool RAND, bool MULTI>
+__device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const bool valid,
+                                                     const AtomsView& a, const TablesView& t,
+                                                     const double4* __restrict__ rtab,
+                                                     const GridView& g, const StepParams& p,
+                                                     const WorkView& w, double2* stage) {
+  const int i = valid ? i_lane : 0;  // lanes past the end read atom 0 and store nothing
+  const int stride = a.stride;
+  const int lane = threadIdx.x & 63;
+  const bool grp = valid && (a.mask[i] & a.groupbit) != 0;
+  const double4 rwi = w.rec[i].rw;
+  const bool on = grp && rwi.w > 0.0;
+  const double4 pi = w.rec[i].p;
+  const double4 xai = w.rec[i].xa;
+  const int ei = MULTI ? static_cast<int>(pi.w) : 0;
+  const int na = on ? w.nact[i] : 0;
+  int nmax = na;
+  for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
+  double fe[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
+  const double2* __restrict__ recs = reinterpret_cast<const double2*>(w.rec);
+  const int q8 = lane >> 3, part = ((lane & 7) - q8) & 7;
+  const int rot = lane & 7;
+  const double2* mine = stage + 72 * (lane >> 3) + 8 * (lane & 7);
+  int jn = 1 < na ? w.act[stride + i] : 0;  // index of entry k + 1
+  if (nmax > 0) coop_fetch(recs, 0 < na ? w.act[i] : 0, q8, part, stage);
+  for (int k = 0; k < nmax; ++k) {
+    // entry k's copies have landed; read this lane's record, then let the
+    // copies of entry k + 1 overwrite the stage
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    wave_lds_sync();
+    const double2 s0 = mine[(0 + rot) & 7], s1 = mine[(1 + rot) & 7], s2 = mine[(2 + rot) & 7];
+    const double2 s3 = mine[(3 + rot) & 7], s4 = mine[(4 + rot) & 7], s5 = mine[(5 + rot) & 7];
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    wave_lds_sync();
+    if (k + 1 < nmax) {
+      coop_fetch(recs, jn, q8, part, stage);
+      jn = k + 2 < na ? w.act[(k + 2) * stride + i] : 0;
+    }
+    const double4 pj = make_double4(s0.x, s0.y, s1.x, s1.y);
+    const double4 rwj = make_double4(s2.x, s2.y, s3.x, s3.y);
+    const double4 xaj = make_double4(s4.x, s4.y, s5.x, s5.y);
+    if (k < na && rwj.w > 0.0)
+      prl_pair<FRIC, RAND, MULTI>(pi, rwi, xai, ei, pj, rwj, xaj, t, rtab, fe, fr);
+  }
+  if (valid) prl_finish<FRIC, RAND>(i, on, pi, fe, fr, a, g, p, w);
+}
+
+// 256 atoms per block, tables through the caches, cooperative gathers
 template <bool FRIC, bool RAND, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
                                                        StepParams p, WorkView w) {
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < a.nlocal) prl_forces_atom<FRIC, RAND, MULTI, true>(i, a, t, t.rho_r2, g, p, w);
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage];
+  const int base = xcd_block() * kBlock;
+  if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
+  const int i = base + threadIdx.x;
+  prl_forces_atom_coop<FRIC, RAND, MULTI>(i, i < a.nlocal, a, t, t.rho_r2, g, p, w,
+                                          s_stage + (threadIdx.x >> 6) * kCoopStage);
 }
 
 template <bool FRIC, bool RAND, bool MULTI, int BLK>
@@ -1100,9 +1222,12 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
                                                              const GridView* __restrict__ grids,
                                                              StepParams p, WorkView w,
                                                              const Chunk* __restrict__ chunks) {
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage];
   const Chunk c = chunks[blockIdx.x];
-  if (static_cast<int>(threadIdx.x) >= c.count) return;
-  prl_forces_atom<FRIC, RAND, MULTI, true>(c.start + threadIdx.x, a, t, t.rho_r2, grids[c.rep], p, w);
+  if (static_cast<int>(threadIdx.x & ~63u) >= c.count) return;  // whole wave past the chunk
+  prl_forces_atom_coop<FRIC, RAND, MULTI>(c.start + threadIdx.x, static_cast<int>(threadIdx.x) < c.count,
+                                          a, t, t.rho_r2, grids[c.rep], p, w,
+                                          s_stage + (threadIdx.x >> 6) * kCoopStage);
 }
 
 // Persistent LDS-table forms: each 1024-thread block stages the rho(r^2)
@@ -1448,7 +1573,7 @@ hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const Grid
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (!fric && !rnd) return hipSuccess;
-  const int f = resolve_form(a, t, form, kLdsForm512);
+  const int f = resolve_form(a, t, form, 0);
   if (f != 0) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int blk = lds_block(f);
