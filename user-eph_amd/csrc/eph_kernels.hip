@@ -43,6 +43,76 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __
   nn[i] = e - b;
 }
 
+// ------------------------------------------------- cooperative gathers
+// A wave fetches the k-th neighbours of its 64 atoms together: the lanes
+// copying one record take consecutive 16-B parts of it, straight into LDS
+// (global_load_lds), so a gather instruction touches one cache line per
+// record instead of one per lane; each lane then reads its own record from
+// LDS.  Every copy lands in a 1 KB block followed by 128 B of padding; inside
+// a record the parts are rotated per record so that the per-lane reads are
+// free of bank conflicts.  The whole wave runs these loops (to its longest
+// list), lanes past their own list doing only copies.
+static __device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+constexpr int kCoopStage = 576;  // 16-B units of LDS per wave
+// density sweep: entries per iteration of the rho loop (144 x 16 B of LDS
+// each) and of the w loop (288 x 16 B each)
+constexpr int kCoopRho = 4, kCoopW = 2;
+constexpr int kCoopStageW = (144 * kCoopRho > 288 * kCoopW) ? 144 * kCoopRho : 288 * kCoopW;
+
+static __device__ __forceinline__ int wave_max(int v) {
+  for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
+  return v;
+}
+
+using lds_void = __attribute__((address_space(3))) void*;
+
+// 32-B position records, 2 lanes per record: in copy m (0, 1) lane 2q + s
+// fetches slot s of the record of lane 32 m + q, which holds part s ^ g(q)
+static __device__ __forceinline__ int pos_rot(int q) { return ((q >> 2) ^ (q >> 3)) & 1; }
+static __device__ __forceinline__ void coop_fetch_pos(const double2* __restrict__ pos2, int j, int lane,
+                                                      double2* stage) {
+  const int q = lane >> 1, part = (lane & 1) ^ pos_rot(lane >> 1);
+  int r0 = __shfl(j, q), r1 = __shfl(j, 32 + q);
+  asm volatile("" : "+v"(r0), "+v"(r1));
+  __builtin_amdgcn_global_load_lds(pos2 + static_cast<size_t>(r0) * 2 + part, (lds_void)(stage + 0), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(pos2 + static_cast<size_t>(r1) * 2 + part, (lds_void)(stage + 72), 16, 0, 0);
+}
+static __device__ __forceinline__ double4 coop_read_pos(const double2* stage, int lane) {
+  const int q = lane & 31, g = pos_rot(q);
+  const double2* row = stage + 72 * (lane >> 5) + 2 * q;
+  const double2 a = row[g], b = row[g ^ 1];
+  return make_double4(a.x, a.y, b.x, b.y);
+}
+
+// 64-B (position, velocity) records, 4 lanes per record: in copy m (0..3)
+// lane 4q + s fetches slot s of the record of lane 16 m + q, which holds part
+// (s - h(q)) mod 4
+static __device__ __forceinline__ int pv_rot(int q) { return ((q >> 1) + (q >> 3)) & 3; }
+static __device__ __forceinline__ void coop_fetch_pv(const double2* __restrict__ pv2, int j, int lane,
+                                                     double2* stage) {
+  const int q = lane >> 2, part = ((lane & 3) - pv_rot(lane >> 2)) & 3;
+  int r0 = __shfl(j, q), r1 = __shfl(j, 16 + q), r2 = __shfl(j, 32 + q), r3 = __shfl(j, 48 + q);
+  asm volatile("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3));
+  const double2* g = pv2 + part;
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r0) * 4, (lds_void)(stage + 0), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r1) * 4, (lds_void)(stage + 72), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r2) * 4, (lds_void)(stage + 144), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r3) * 4, (lds_void)(stage + 216), 16, 0, 0);
+}
+static __device__ __forceinline__ void coop_read_pv(const double2* stage, int lane, double4* pj,
+                                                    double4* vj) {
+  const int q = lane & 15, h = pv_rot(q);
+  const double2* row = stage + 72 * (lane >> 4) + 4 * q;
+  const double2 a = row[h], b = row[(1 + h) & 3], c = row[(2 + h) & 3], d = row[(3 + h) & 3];
+  *pj = make_double4(a.x, a.y, b.x, b.y);
+  *vj = make_double4(c.x, c.y, d.x, d.y);
+}
+
 // ------------------------------------------------- density + w (stage 1)
 // FixEPH::calculate_environment (fix_eph.cpp:431-466) followed by stage 1
 // of force_prl (fix_eph.cpp:702-741) and the noise draw of
@@ -50,6 +120,78 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __
 // r^2 < rc^2 into an active list, which the w sweep and the force sweep
 // re-read in the same order (they skip exactly the other entries).
 // `rtab` holds the rho(r^2) tables, in global memory or staged in LDS.
+// alpha(rho_i) (eph_beta.h:164-198 via fix_eph.cpp:711); rho above the
+// table's range gives 0 and raises the warning flag (eph_beta.h:189-194)
+template <bool MULTI>
+__device__ __forceinline__ double density_alpha(const bool grp, const double rho, const double4& pi,
+                                                const TablesView& t, const WorkView& w) {
+  double alpha = 0.0;
+  if (grp && rho > 0.0) {
+    if (rho > t.rho_cut) {
+      atomicOr(w.flags_out, 1);  // eph_beta.h:189-194: alpha = 0 with a warning
+    } else {
+      const int ei = MULTI ? static_cast<int>(pi.w) : 0;
+      alpha = cubic_at(t.alpha + ei * t.n_beta, t.n_beta, t.inv_drho, rho);
+    }
+  }
+  return alpha;
+}
+
+// one active neighbour's term of w_i (fix_eph.cpp:719-737)
+template <bool MULTI>
+__device__ __forceinline__ void density_w_pair(const double4& pi, const double4& vi, const double4& pj,
+                                               const double4& vj, const double rho, const double alpha,
+                                               const TablesView& t, const double4* __restrict__ rtab,
+                                               double* wx, double* wy, double* wz) {
+  const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+  const double r2 = ex * ex + ey * ey + ez * ez;
+  const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+  const double vr = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+  const double pre = alpha * vr / (rho * r2);
+  const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
+  const double var1 = pre * ev1;
+  const double ev2 = ex * vj.x + ey * vj.y + ez * vj.z;
+  const double var2 = pre * ev2;
+  const double dvar = var1 - var2;
+  *wx += dvar * ex;
+  *wy += dvar * ey;
+  *wz += dvar * ez;
+}
+
+// the noise xi_i (drawn, injected or none), the v1 filter, and atom i's record
+template <int XI>
+__device__ __forceinline__ void density_finish(const int i, const bool grp, const double4& pi,
+                                               const double rho, const double alpha, const double wx,
+                                               const double wy, const double wz, const AtomsView& a,
+                                               const StepParams& p, const WorkView& w) {
+  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
+  if (XI == 1 && grp) {
+    x0 = p.xi_in[3 * i + 0];
+    x1 = p.xi_in[3 * i + 1];
+    x2 = p.xi_in[3 * i + 2];
+  } else if (XI == 2 && grp) {
+    double g3[3];
+    gaussian3(p.seed, static_cast<uint32_t>(a.tag[i]), p.step, g3);
+    x0 = g3[0]; x1 = g3[1]; x2 = g3[2];
+  }
+  if (XI != 0 && p.colour == 3) {
+    // eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp:645-652): every local
+    // atom's noise is filtered, zi <- zi (1 - zeta) + zeta xi, and the random
+    // sums use zi
+    const double om = 1. - p.zeta;
+    double4 z = w.cB[i];
+    z.x = z.x * om + p.zeta * x0;
+    z.y = z.y * om + p.zeta * x1;
+    z.z = z.z * om + p.zeta * x2;
+    w.cB[i] = z;
+    x0 = z.x; x1 = z.y; x2 = z.z;
+  }
+  ForceRec* r = w.rec + i;
+  r->p = pi;
+  r->rw = make_double4(wx, wy, wz, rho);
+  r->xa = make_double4(x0, x1, x2, alpha);
+}
+
 template <int XI, bool FRIC, bool MULTI, bool X2>
 __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, const TablesView& t,
                                                const double4* __restrict__ rtab,
@@ -120,15 +262,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
     }
   }
   w.nact[i] = na;
-  double alpha = 0.0;
-  if (grp && rho > 0.0) {
-    if (rho > t.rho_cut) {
-      atomicOr(w.flags_out, 1);  // eph_beta.h:189-194: alpha = 0 with a warning
-    } else {
-      const int ei = MULTI ? static_cast<int>(pi.w) : 0;
-      alpha = cubic_at(t.alpha + ei * t.n_beta, t.n_beta, t.inv_drho, rho);
-    }
-  }
+  const double alpha = density_alpha<MULTI>(grp, rho, pi, t, w);
   double wx = 0.0, wy = 0.0, wz = 0.0;
   if (FRIC && grp && rho > 0.0) {
     const double4 vi = w.vw[i];
@@ -145,54 +279,130 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
         pn = w.pv[jn].p;
         vn = w.pv[jn].v;
       }
-      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-      const double r2 = ex * ex + ey * ey + ez * ez;
-      const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-      const double vr = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
-      const double pre = alpha * vr / (rho * r2);
-      const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
-      const double var1 = pre * ev1;
-      const double ev2 = ex * vj.x + ey * vj.y + ez * vj.z;
-      const double var2 = pre * ev2;
-      const double dvar = var1 - var2;
-      wx += dvar * ex;
-      wy += dvar * ey;
-      wz += dvar * ez;
+      density_w_pair<MULTI>(pi, vi, pj, vj, rho, alpha, t, rtab, &wx, &wy, &wz);
     }
   }
-  double x0 = 0.0, x1 = 0.0, x2 = 0.0;
-  if (XI == 1 && grp) {
-    x0 = p.xi_in[3 * i + 0];
-    x1 = p.xi_in[3 * i + 1];
-    x2 = p.xi_in[3 * i + 2];
-  } else if (XI == 2 && grp) {
-    double g3[3];
-    gaussian3(p.seed, static_cast<uint32_t>(a.tag[i]), p.step, g3);
-    x0 = g3[0]; x1 = g3[1]; x2 = g3[2];
+  density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
+}
+
+// Cooperative form of density_w_atom (same sums in the same order): the
+// positions of the rho loop and the (position, velocity) records of the w loop
+// come through LDS, see coop_fetch_pos / coop_fetch_pv.  The records of entry
+// k + 1 are in flight while entry k is computed.  `valid` = lane i is a local
+// atom; stage: kCoopStageW x 16 B of LDS owned by the calling wave.
+template <int XI, bool FRIC, bool MULTI>
+__device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool valid, const AtomsView& a,
+                                                    const TablesView& t, const double4* __restrict__ rtab,
+                                                    const StepParams& p, const WorkView& w,
+                                                    double2* stage) {
+  const int i = valid ? i_lane : 0;  // lanes past the end read atom 0 and store nothing
+  const int stride = a.stride;
+  const int lane = threadIdx.x & 63;
+  const double4 pi = a.pos[i];
+  const bool grp = valid && (a.mask[i] & a.groupbit) != 0;
+  const int nn = grp ? w.nnbr[i] : 0;
+  const int nnmax = wave_max(nn);
+  double rho = 0.0;
+  int na = 0;
+  {
+    const double2* __restrict__ pos2 = reinterpret_cast<const double2*>(a.pos);
+    // kCoopRho entries per iteration: the positions of entry k + e land at
+    // stage + 144 e, and those of the next kCoopRho entries are in flight while
+    // these are summed (in list order)
+    constexpr int U = kCoopRho;
+    int jj[U], nx[U];
+#pragma unroll
+    for (int e = 0; e < U; ++e) jj[e] = e < nn ? w.nbr[e * stride + i] : 0;
+#pragma unroll
+    for (int e = 0; e < U; ++e) nx[e] = U + e < nn ? w.nbr[(U + e) * stride + i] : 0;
+    if (nnmax > 0) {
+#pragma unroll
+      for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, jj[e], lane, stage + 144 * e);
+    }
+    for (int k = 0; k < nnmax; k += U) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      wave_lds_sync();
+      double4 pq[U];
+#pragma unroll
+      for (int e = 0; e < U; ++e) pq[e] = coop_read_pos(stage + 144 * e, lane);
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      wave_lds_sync();
+      int jc[U];
+#pragma unroll
+      for (int e = 0; e < U; ++e) jc[e] = jj[e];
+      if (k + U < nnmax) {
+#pragma unroll
+        for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, nx[e], lane, stage + 144 * e);
+#pragma unroll
+        for (int e = 0; e < U; ++e) {
+          jj[e] = nx[e];
+          nx[e] = k + 2 * U + e < nn ? w.nbr[(k + 2 * U + e) * stride + i] : 0;
+        }
+      }
+#pragma unroll
+      for (int e = 0; e < U; ++e) {
+        if (k + e < nn) {
+          const double ex = pq[e].x - pi.x, ey = pq[e].y - pi.y, ez = pq[e].z - pi.z;
+          const double r2 = ex * ex + ey * ey + ez * ez;
+          if (r2 < t.rc2) {
+            const int ej = MULTI ? static_cast<int>(pq[e].w) : 0;
+            rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+            w.act[na * stride + i] = jc[e];
+            ++na;
+          }
+        }
+      }
+    }
   }
-  if (XI != 0 && p.colour == 3) {
-    // eph/coloured/exp/v1 (fix_eph_coloured_exp_v1.cpp:645-652): every local
-    // atom's noise is filtered, zi <- zi (1 - zeta) + zeta xi, and the random
-    // sums use zi
-    const double om = 1. - p.zeta;
-    double4 z = w.cB[i];
-    z.x = z.x * om + p.zeta * x0;
-    z.y = z.y * om + p.zeta * x1;
-    z.z = z.z * om + p.zeta * x2;
-    w.cB[i] = z;
-    x0 = z.x; x1 = z.y; x2 = z.z;
+  if (valid) w.nact[i] = na;
+  const double alpha = density_alpha<MULTI>(grp, rho, pi, t, w);
+  double wx = 0.0, wy = 0.0, wz = 0.0;
+  if (FRIC) {
+    const int nw = (grp && rho > 0.0) ? na : 0;
+    const int nwmax = wave_max(nw);
+    const double4 vi = w.vw[i];
+    const double2* __restrict__ pv2 = reinterpret_cast<const double2*>(w.pv);
+    // kCoopW entries per iteration, (position, velocity) of entry k + e at
+    // stage + 288 e
+    constexpr int U = kCoopW;
+    int jj[U];
+#pragma unroll
+    for (int e = 0; e < U; ++e) jj[e] = U + e < nw ? w.act[(U + e) * stride + i] : 0;
+    if (nwmax > 0) {
+#pragma unroll
+      for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, e < nw ? w.act[e * stride + i] : 0, lane, stage + 288 * e);
+    }
+    for (int k = 0; k < nwmax; k += U) {
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+      wave_lds_sync();
+      double4 pq[U], vq[U];
+#pragma unroll
+      for (int e = 0; e < U; ++e) coop_read_pv(stage + 288 * e, lane, &pq[e], &vq[e]);
+      asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+      wave_lds_sync();
+      if (k + U < nwmax) {
+#pragma unroll
+        for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, jj[e], lane, stage + 288 * e);
+#pragma unroll
+        for (int e = 0; e < U; ++e) jj[e] = k + 2 * U + e < nw ? w.act[(k + 2 * U + e) * stride + i] : 0;
+      }
+#pragma unroll
+      for (int e = 0; e < U; ++e)
+        if (k + e < nw) density_w_pair<MULTI>(pi, vi, pq[e], vq[e], rho, alpha, t, rtab, &wx, &wy, &wz);
+    }
   }
-  ForceRec* r = w.rec + i;
-  r->p = pi;
-  r->rw = make_double4(wx, wy, wz, rho);
-  r->xa = make_double4(x0, x1, x2, alpha);
+  if (valid) density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
 }
 
 template <int XI, bool FRIC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
                                                       WorkView w) {
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI, true>(i, a, t, t.rho_r2, p, w);
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStageW];
+  const int base = xcd_block() * kBlock;
+  if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
+  const int i = base + threadIdx.x;
+  density_w_atom_coop<XI, FRIC, MULTI>(i, i < a.nlocal, a, t, t.rho_r2, p, w,
+                                       s_stage + (threadIdx.x >> 6) * kCoopStageW);
 }
 
 // Persistent form: each block stages the rho(r^2) tables in LDS once and then
@@ -417,13 +627,6 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
 // the wave's longest active list (lanes past their own do only the copies),
 // so the whole wave must call this; `valid` = lane i is a local atom.
 // stage: 576 x 16 B of LDS owned by the calling wave.
-static __device__ __forceinline__ void wave_lds_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-constexpr int kCoopStage = 576;  // 16-B units of LDS per wave
 
 static __device__ __forceinline__ void coop_fetch(const double2* __restrict__ recs, int j, int q8,
                                                   int part, double2* stage) {
