@@ -58,7 +58,8 @@ static __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kCoopStage = 576;  // 16-B units of LDS per wave
+constexpr int kCoopStage = 576;  // 16-B units of LDS per wave and force-sweep entry
+constexpr int kCoopF = 1;        // force-sweep entries per iteration
 // density sweep: entries per iteration of the rho loop (144 x 16 B of LDS
 // each) and of the w loop (288 x 16 B each)
 constexpr int kCoopRho = 4, kCoopW = 2;
@@ -192,7 +193,9 @@ __device__ __forceinline__ void density_finish(const int i, const bool grp, cons
   r->xa = make_double4(x0, x1, x2, alpha);
 }
 
-template <int XI, bool FRIC, bool MULTI, bool X2>
+// Per-thread form (used by the LDS-table kernels): each thread gathers its own
+// neighbours, one iteration ahead of their use.
+template <int XI, bool FRIC, bool MULTI>
 __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, const TablesView& t,
                                                const double4* __restrict__ rtab,
                                                const StepParams& p, const WorkView& w) {
@@ -201,7 +204,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   const bool grp = (a.mask[i] & a.groupbit) != 0;
   double rho = 0.0;
   int na = 0;
-  if (grp && !X2) {
+  if (grp) {
     // each neighbour's position is loaded one iteration ahead of its use
     const int nn = w.nnbr[i];
     int jn = nn > 0 ? w.nbr[i] : 0;
@@ -219,44 +222,6 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
         const int ej = MULTI ? static_cast<int>(pj.w) : 0;
         rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
         w.act[na * stride + i] = j;
-        ++na;
-      }
-    }
-  }
-  if (grp && X2) {
-    // Two entries per step: their r^2 and table values are independent, and
-    // are added to rho in list order.  The positions of the next two entries
-    // and the indices of the two after them are in flight meanwhile.
-    const int nn = w.nnbr[i];
-    const int* __restrict__ L = w.nbr + i;
-    int ja = nn > 0 ? L[0] : 0, jb = nn > 1 ? L[stride] : 0;
-    double4 qa = nn > 0 ? a.pos[ja] : pi, qb = nn > 1 ? a.pos[jb] : pi;
-    int ka = nn > 2 ? L[2 * stride] : 0, kb = nn > 3 ? L[3 * stride] : 0;
-    for (int k = 0; k < nn; k += 2) {
-      const int j0 = ja, j1 = jb;
-      const double4 p0 = qa, p1 = qb;
-      ja = ka; jb = kb;
-      if (k + 2 < nn) qa = a.pos[ka];
-      if (k + 3 < nn) qb = a.pos[kb];
-      if (k + 4 < nn) ka = L[(k + 4) * stride];
-      if (k + 5 < nn) kb = L[(k + 5) * stride];
-      const double ax = p0.x - pi.x, ay = p0.y - pi.y, az = p0.z - pi.z;
-      const double bx = p1.x - pi.x, by = p1.y - pi.y, bz = p1.z - pi.z;
-      const double r2a = ax * ax + ay * ay + az * az;
-      const double r2b = bx * bx + by * by + bz * bz;
-      const bool ua = r2a < t.rc2, ub = k + 1 < nn && r2b < t.rc2;
-      const int ea = MULTI ? static_cast<int>(p0.w) : 0;
-      const int eb = MULTI ? static_cast<int>(p1.w) : 0;
-      const double ta = cubic_at(rtab + ea * t.n_rho, t.n_rho, t.inv_dr2, r2a);
-      const double tb = cubic_at(rtab + eb * t.n_rho, t.n_rho, t.inv_dr2, r2b);
-      if (ua) {
-        rho += ta;
-        w.act[na * stride + i] = j0;
-        ++na;
-      }
-      if (ub) {
-        rho += tb;
-        w.act[na * stride + i] = j1;
         ++na;
       }
     }
@@ -394,6 +359,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   if (valid) density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
 }
 
+// 256 atoms per block, tables through the caches, cooperative gathers
 template <int XI, bool FRIC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
                                                       WorkView w) {
@@ -440,7 +406,7 @@ __global__ __launch_bounds__(BLK) void k_density_w_lds(AtomsView a, TablesView t
   const int end = xcd_tiles_end(ntiles);
   for (; tile < end; tile += step) {
     const int i = tile * BLK + threadIdx.x;
-    if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI, false>(i, a, t, s_tab, p, w);
+    if (i < a.nlocal) density_w_atom<XI, FRIC, MULTI>(i, a, t, s_tab, p, w);
   }
 }
 
@@ -672,26 +638,43 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
   const int q8 = lane >> 3, part = ((lane & 7) - q8) & 7;
   const int rot = lane & 7;
   const double2* mine = stage + 72 * (lane >> 3) + 8 * (lane & 7);
-  int jn = 1 < na ? w.act[stride + i] : 0;  // index of entry k + 1
-  if (nmax > 0) coop_fetch(recs, 0 < na ? w.act[i] : 0, q8, part, stage);
-  for (int k = 0; k < nmax; ++k) {
-    // entry k's copies have landed; read this lane's record, then let the
-    // copies of entry k + 1 overwrite the stage
+  // kCoopF entries per iteration, the records of entry k + e at
+  // stage + kCoopStage e
+  constexpr int U = kCoopF;
+  int jj[U];
+#pragma unroll
+  for (int e = 0; e < U; ++e) jj[e] = U + e < na ? w.act[(U + e) * stride + i] : 0;
+  if (nmax > 0) {
+#pragma unroll
+    for (int e = 0; e < U; ++e)
+      coop_fetch(recs, e < na ? w.act[e * stride + i] : 0, q8, part, stage + kCoopStage * e);
+  }
+  for (int k = 0; k < nmax; k += U) {
+    // entries k.. have landed; read this lane's records, then let the copies
+    // of the next entries overwrite the stage
     asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
     wave_lds_sync();
-    const double2 s0 = mine[(0 + rot) & 7], s1 = mine[(1 + rot) & 7], s2 = mine[(2 + rot) & 7];
-    const double2 s3 = mine[(3 + rot) & 7], s4 = mine[(4 + rot) & 7], s5 = mine[(5 + rot) & 7];
+    double2 sv[U][6];
+#pragma unroll
+    for (int e = 0; e < U; ++e)
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sv[e][c] = mine[kCoopStage * e + ((c + rot) & 7)];
     asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
     wave_lds_sync();
-    if (k + 1 < nmax) {
-      coop_fetch(recs, jn, q8, part, stage);
-      jn = k + 2 < na ? w.act[(k + 2) * stride + i] : 0;
+    if (k + U < nmax) {
+#pragma unroll
+      for (int e = 0; e < U; ++e) coop_fetch(recs, jj[e], q8, part, stage + kCoopStage * e);
+#pragma unroll
+      for (int e = 0; e < U; ++e) jj[e] = k + 2 * U + e < na ? w.act[(k + 2 * U + e) * stride + i] : 0;
     }
-    const double4 pj = make_double4(s0.x, s0.y, s1.x, s1.y);
-    const double4 rwj = make_double4(s2.x, s2.y, s3.x, s3.y);
-    const double4 xaj = make_double4(s4.x, s4.y, s5.x, s5.y);
-    if (k < na && rwj.w > 0.0)
-      prl_pair<FRIC, RAND, MULTI>(pi, rwi, xai, ei, pj, rwj, xaj, t, rtab, fe, fr);
+#pragma unroll
+    for (int e = 0; e < U; ++e) {
+      const double4 pj = make_double4(sv[e][0].x, sv[e][0].y, sv[e][1].x, sv[e][1].y);
+      const double4 rwj = make_double4(sv[e][2].x, sv[e][2].y, sv[e][3].x, sv[e][3].y);
+      const double4 xaj = make_double4(sv[e][4].x, sv[e][4].y, sv[e][5].x, sv[e][5].y);
+      if (k + e < na && rwj.w > 0.0)
+        prl_pair<FRIC, RAND, MULTI>(pi, rwi, xai, ei, pj, rwj, xaj, t, rtab, fe, fr);
+    }
   }
   if (valid) prl_finish<FRIC, RAND>(i, on, pi, fe, fr, a, g, p, w);
 }
@@ -700,12 +683,12 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
 template <bool FRIC, bool RAND, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
                                                        StepParams p, WorkView w) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage];
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage * kCoopF];
   const int base = xcd_block() * kBlock;
   if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
   const int i = base + threadIdx.x;
   prl_forces_atom_coop<FRIC, RAND, MULTI>(i, i < a.nlocal, a, t, t.rho_r2, g, p, w,
-                                          s_stage + (threadIdx.x >> 6) * kCoopStage);
+                                          s_stage + (threadIdx.x >> 6) * kCoopStage * kCoopF);
 }
 
 template <bool FRIC, bool RAND, bool MULTI, int BLK>
@@ -1399,12 +1382,14 @@ __global__ __launch_bounds__(kBlock) void k_density_w_batch(AtomsView a, TablesV
                                                             const Chunk* __restrict__ chunks,
                                                             const uint32_t* __restrict__ seeds,
                                                             const uint64_t* __restrict__ step) {
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStageW];
   const Chunk c = chunks[blockIdx.x];
-  if (static_cast<int>(threadIdx.x) >= c.count) return;
+  if (static_cast<int>(threadIdx.x & ~63u) >= c.count) return;  // whole wave past the chunk
   StepParams pr = p;
   pr.seed = seeds[c.rep];
   pr.step = *step;
-  density_w_atom<XI, FRIC, MULTI, true>(c.start + threadIdx.x, a, t, t.rho_r2, pr, w);
+  density_w_atom_coop<XI, FRIC, MULTI>(c.start + threadIdx.x, static_cast<int>(threadIdx.x) < c.count,
+                                       a, t, t.rho_r2, pr, w, s_stage + (threadIdx.x >> 6) * kCoopStageW);
 }
 
 // start of a recorded step: step += 1
@@ -1425,12 +1410,12 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
                                                              const GridView* __restrict__ grids,
                                                              StepParams p, WorkView w,
                                                              const Chunk* __restrict__ chunks) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage];
+  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage * kCoopF];
   const Chunk c = chunks[blockIdx.x];
   if (static_cast<int>(threadIdx.x & ~63u) >= c.count) return;  // whole wave past the chunk
   prl_forces_atom_coop<FRIC, RAND, MULTI>(c.start + threadIdx.x, static_cast<int>(threadIdx.x) < c.count,
                                           a, t, t.rho_r2, grids[c.rep], p, w,
-                                          s_stage + (threadIdx.x >> 6) * kCoopStage);
+                                          s_stage + (threadIdx.x >> 6) * kCoopStage * kCoopF);
 }
 
 // Persistent LDS-table forms: each 1024-thread block stages the rho(r^2)
@@ -1463,7 +1448,7 @@ __global__ __launch_bounds__(kBigBlock) void k_density_w_batch_lds(AtomsView a, 
     const Chunk c = chunks[ci];
     if (lane >= c.count) continue;
     pr.seed = seeds[c.rep];
-    density_w_atom<XI, FRIC, MULTI, false>(c.start + lane, a, t, s_tab, pr, w);
+    density_w_atom<XI, FRIC, MULTI>(c.start + lane, a, t, s_tab, pr, w);
   }
 }
 
@@ -1715,7 +1700,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
-  const int f = resolve_form(a, t, form, kLdsForm1024);
+  const int f = resolve_form(a, t, form, 0);
   if (f != 0) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int blk = lds_block(f);
