@@ -2,6 +2,9 @@
 
 * an empty system (no atoms) runs every hook and deposits nothing;
 * an isolated atom (rho = 0) and an isolated dimer against the oracle;
+* list lengths that differ widely within a wavefront (a dense cluster in a
+  dilute gas, a partial last wavefront, half the atoms outside the group)
+  against the oracle;
 * a list index out of range is rejected (and the list dropped);
 * the host-array hooks' staged copies round-trip a 400,003-atom system;
 * the 2,048,000-atom four-species cascade lattice (fcc 80^3, 10 keV PKA, the
@@ -76,6 +79,50 @@ def test_isolated_atom_and_dimer_match_oracle(oracle, tmp_path):
             g, o = gpu.get(q)[:3], ora.get(q, 3)
             assert np.array_equal(g, o), q
         assert np.array_equal(sB.f[:3], sA.f[:3])
+        assert np.array_equal(gpu.grid().get("T"), ora.fdm().get("T"))
+
+
+def test_ragged_neighbour_counts_match_oracle(oracle, tmp_path):
+    """The cooperative sweeps run every wavefront to its longest list, the lanes
+    with shorter lists only fetching; each atom must still match the oracle
+    bit for bit."""
+    rng = np.random.default_rng(41)
+    L = 36.0
+    cluster = minisys.fcc(3, 3.615) + 2.0  # 108 atoms in [2, 13)^3, ~42 active each
+    g = np.arange(3.0, L, 6.0)
+    grid = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3)
+    grid = grid[~np.all(grid < 16.0, axis=1)]  # dilute gas outside the cluster's corner
+    gas = grid + rng.uniform(-0.5, 0.5, grid.shape)
+    x = np.concatenate([cluster, gas])
+    order = rng.permutation(len(x))  # clustered and gas atoms share wavefronts
+    x = x[order]
+    n = len(x)
+    assert n % 64 != 0
+    v = rng.normal(0.0, 3.0, x.shape)
+    types = 1 + np.arange(n) % 2
+    mask = np.where(rng.uniform(size=n) < 0.5, 3, 4)
+    sA = minisys.System(x, v, types, [0, 0, 0], [L] * 3, 6.5, None, mask=mask)
+    sB = minisys.System(x, v, types, [0, 0, 0], [L] * 3, 6.5, None, mask=mask)
+    cnt = np.diff(sB.nbr_off)
+    assert cnt.max() > 40 and cnt.min() < 10
+    p = str(tmp_path / "r.beta")
+    names = inputs.workload_beta(p, 2)
+    args = inputs.fix_args(p, names, grid=(4, 4, 4), seed=9)
+    masses = [inputs.MASS_CU] * 2
+    ora = oracle.Fix(args, 2, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V, masses=masses)
+    gpu = ea.FixEPH(args, 2, [0, 0, 0], [L] * 3, 1e-3, inputs.BOLTZ, inputs.FTM2V, masses=masses)
+    ora.set_groupbit(2)
+    gpu.set_groupbit(2)
+    gpu.init()
+    gpu.set_atoms(sB)
+    for step in (1, 2, 3):
+        ora.post_force(sA, step=step)
+        gpu.post_force(sB, step=step)
+        ora.end_of_step(sA, step)
+        gpu.end_of_step(sB, step)
+        for q in ("rho", "f_EPH", "f_RNG"):
+            assert np.array_equal(gpu.get(q)[:n], ora.get(q, n)), (q, step)
+        assert np.array_equal(sB.f[:n], sA.f[:n])
         assert np.array_equal(gpu.grid().get("T"), ora.fdm().get("T"))
 
 

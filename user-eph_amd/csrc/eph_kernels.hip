@@ -48,10 +48,10 @@ __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __
 // copying one record take consecutive 16-B parts of it, straight into LDS
 // (global_load_lds), so a gather instruction touches one cache line per
 // record instead of one per lane; each lane then reads its own record from
-// LDS.  Every copy lands in a 1 KB block followed by 128 B of padding; inside
-// a record the parts are rotated per record so that the per-lane reads are
-// free of bank conflicts.  The whole wave runs these loops (to its longest
-// list), lanes past their own list doing only copies.
+// LDS.  Every copy fills a 1 KB block (the force sweep's add 128 B of
+// padding); inside a record the parts are rotated per record so that the
+// per-lane reads are free of bank conflicts.  The whole wave runs these loops
+// (to its longest list), lanes past their own list doing only copies.
 static __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
@@ -60,10 +60,11 @@ static __device__ __forceinline__ void wave_lds_sync() {
 
 constexpr int kCoopStage = 576;  // 16-B units of LDS per wave and force-sweep entry
 constexpr int kCoopF = 1;        // force-sweep entries per iteration
-// density sweep: entries per iteration of the rho loop (144 x 16 B of LDS
-// each) and of the w loop (288 x 16 B each)
+// density sweep: entries per iteration of the rho loop (128 x 16 B of LDS
+// each) and of the w loop (256 x 16 B each); their copies need no padding,
+// since lanes reading from different copies are never served together
 constexpr int kCoopRho = 4, kCoopW = 2;
-constexpr int kCoopStageW = (144 * kCoopRho > 288 * kCoopW) ? 144 * kCoopRho : 288 * kCoopW;
+constexpr int kCoopStageW = (128 * kCoopRho > 256 * kCoopW) ? 128 * kCoopRho : 256 * kCoopW;
 
 static __device__ __forceinline__ int wave_max(int v) {
   for (int o = 32; o > 0; o >>= 1) v = max(v, __shfl_xor(v, o));
@@ -81,11 +82,11 @@ static __device__ __forceinline__ void coop_fetch_pos(const double2* __restrict_
   int r0 = __shfl(j, q), r1 = __shfl(j, 32 + q);
   asm volatile("" : "+v"(r0), "+v"(r1));
   __builtin_amdgcn_global_load_lds(pos2 + static_cast<size_t>(r0) * 2 + part, (lds_void)(stage + 0), 16, 0, 0);
-  __builtin_amdgcn_global_load_lds(pos2 + static_cast<size_t>(r1) * 2 + part, (lds_void)(stage + 72), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(pos2 + static_cast<size_t>(r1) * 2 + part, (lds_void)(stage + 64), 16, 0, 0);
 }
 static __device__ __forceinline__ double4 coop_read_pos(const double2* stage, int lane) {
   const int q = lane & 31, g = pos_rot(q);
-  const double2* row = stage + 72 * (lane >> 5) + 2 * q;
+  const double2* row = stage + 64 * (lane >> 5) + 2 * q;
   const double2 a = row[g], b = row[g ^ 1];
   return make_double4(a.x, a.y, b.x, b.y);
 }
@@ -101,14 +102,14 @@ static __device__ __forceinline__ void coop_fetch_pv(const double2* __restrict__
   asm volatile("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3));
   const double2* g = pv2 + part;
   __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r0) * 4, (lds_void)(stage + 0), 16, 0, 0);
-  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r1) * 4, (lds_void)(stage + 72), 16, 0, 0);
-  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r2) * 4, (lds_void)(stage + 144), 16, 0, 0);
-  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r3) * 4, (lds_void)(stage + 216), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r1) * 4, (lds_void)(stage + 64), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r2) * 4, (lds_void)(stage + 128), 16, 0, 0);
+  __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r3) * 4, (lds_void)(stage + 192), 16, 0, 0);
 }
 static __device__ __forceinline__ void coop_read_pv(const double2* stage, int lane, double4* pj,
                                                     double4* vj) {
   const int q = lane & 15, h = pv_rot(q);
-  const double2* row = stage + 72 * (lane >> 4) + 4 * q;
+  const double2* row = stage + 64 * (lane >> 4) + 4 * q;
   const double2 a = row[h], b = row[(1 + h) & 3], c = row[(2 + h) & 3], d = row[(3 + h) & 3];
   *pj = make_double4(a.x, a.y, b.x, b.y);
   *vj = make_double4(c.x, c.y, d.x, d.y);
@@ -272,7 +273,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   {
     const double2* __restrict__ pos2 = reinterpret_cast<const double2*>(a.pos);
     // kCoopRho entries per iteration: the positions of entry k + e land at
-    // stage + 144 e, and those of the next kCoopRho entries are in flight while
+    // stage + 128 e, and those of the next kCoopRho entries are in flight while
     // these are summed (in list order)
     constexpr int U = kCoopRho;
     int jj[U], nx[U];
@@ -282,14 +283,14 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
     for (int e = 0; e < U; ++e) nx[e] = U + e < nn ? w.nbr[(U + e) * stride + i] : 0;
     if (nnmax > 0) {
 #pragma unroll
-      for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, jj[e], lane, stage + 144 * e);
+      for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, jj[e], lane, stage + 128 * e);
     }
     for (int k = 0; k < nnmax; k += U) {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       wave_lds_sync();
       double4 pq[U];
 #pragma unroll
-      for (int e = 0; e < U; ++e) pq[e] = coop_read_pos(stage + 144 * e, lane);
+      for (int e = 0; e < U; ++e) pq[e] = coop_read_pos(stage + 128 * e, lane);
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       wave_lds_sync();
       int jc[U];
@@ -297,7 +298,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
       for (int e = 0; e < U; ++e) jc[e] = jj[e];
       if (k + U < nnmax) {
 #pragma unroll
-        for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, nx[e], lane, stage + 144 * e);
+        for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, nx[e], lane, stage + 128 * e);
 #pragma unroll
         for (int e = 0; e < U; ++e) {
           jj[e] = nx[e];
@@ -328,26 +329,26 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
     const double4 vi = w.vw[i];
     const double2* __restrict__ pv2 = reinterpret_cast<const double2*>(w.pv);
     // kCoopW entries per iteration, (position, velocity) of entry k + e at
-    // stage + 288 e
+    // stage + 256 e
     constexpr int U = kCoopW;
     int jj[U];
 #pragma unroll
     for (int e = 0; e < U; ++e) jj[e] = U + e < nw ? w.act[(U + e) * stride + i] : 0;
     if (nwmax > 0) {
 #pragma unroll
-      for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, e < nw ? w.act[e * stride + i] : 0, lane, stage + 288 * e);
+      for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, e < nw ? w.act[e * stride + i] : 0, lane, stage + 256 * e);
     }
     for (int k = 0; k < nwmax; k += U) {
       asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       wave_lds_sync();
       double4 pq[U], vq[U];
 #pragma unroll
-      for (int e = 0; e < U; ++e) coop_read_pv(stage + 288 * e, lane, &pq[e], &vq[e]);
+      for (int e = 0; e < U; ++e) coop_read_pv(stage + 256 * e, lane, &pq[e], &vq[e]);
       asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
       wave_lds_sync();
       if (k + U < nwmax) {
 #pragma unroll
-        for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, jj[e], lane, stage + 288 * e);
+        for (int e = 0; e < U; ++e) coop_fetch_pv(pv2, jj[e], lane, stage + 256 * e);
 #pragma unroll
         for (int e = 0; e < U; ++e) jj[e] = k + 2 * U + e < nw ? w.act[(k + 2 * U + e) * stride + i] : 0;
       }
