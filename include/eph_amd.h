@@ -194,6 +194,16 @@ void* eph_stream(eph_context* ctx);
  * keeps the fastest; forms[2] receives the choice (density, force) and
  * ms[2 * EPH_SWEEP_FORMS] the times (density forms 0..3, then force forms
  * 0..3; 0 for a form the device cannot run).  f is not changed. */
+/* Candidate list of the density sweep (no counterpart in the reference; the
+ * results are the same bits either way).  In form 0 the rho loop walks the
+ * entries of the full list that were within r_cutoff + delta when the list
+ * was last scanned, as long as no local atom has moved 0.45 delta since; the
+ * entries it skips are then beyond r_cutoff and inactive.  Otherwise it scans
+ * the full list and retakes the candidates.  delta < 0: 0.1 r_cutoff (the
+ * default), 0: always the full list.  Ghost atoms must be images of local
+ * atoms with fixed shifts between list uploads, as eph_set_ghost_map states. */
+int eph_set_candidate_margin(eph_context* ctx, double delta);
+
 #define EPH_SWEEP_FORMS 4
 int eph_set_sweep_forms(eph_context* ctx, int density, int force);
 int eph_autotune(eph_context* ctx, int* forms, double* ms);

@@ -80,6 +80,20 @@ def test_trajectory_fcc10_100_steps(oracle, tmp_path):
     assert abs(p.gpu.compute_vector(1) - p.ora.compute_vector(1)) <= 1e-12 * p.ora.compute_vector(1)
 
 
+@pytest.mark.parametrize("delta", [0.0, 1e-3, 0.4, -1.0])
+def test_candidate_margin_keeps_bits(oracle, tmp_path, delta):
+    """The density sweep's rho loop over the candidate list gives the oracle's
+    bits for any margin: none (full list every step), one so small that the
+    candidates are retaken nearly every step, a wide one and the default, over
+    a 60-step 900 K trajectory with list rebuilds."""
+    p = Pair(oracle, ea, str(tmp_path), cells=6, T=900.0, grid=6, dt=1e-3)
+    p.gpu.set_candidate_margin(delta)
+    for k in range(60):
+        p.step()
+        if k % 20 == 19:
+            _fix_state_close(p)
+
+
 @pytest.mark.parametrize("forms", [("auto", "auto"), ("lds", "lds")])
 def test_alloy_cascade_small(oracle, tmp_path, forms):
     """4 species, 1 keV PKA, hot spot, dt 0.1 fs: multi-element tables and

@@ -178,6 +178,7 @@ int eph_adjoint::restore(const State& st) {
   HCPY(ctx->scal.p, st.scal.p, 2 * sizeof(double));
   HCPY(ctx->nbr.p, st.nbr.p, sf * sizeof(int));
   HCPY(ctx->nnbr.p, st.nnbr.p, ctx->stride * sizeof(int));
+  ctx->drop_candidates();
   HCPY(md->lj_nbr.p, st.lj.p, sl * sizeof(int));
   HCPY(md->lj_nn.p, st.ljn.p, ctx->stride * sizeof(int));
   EPH_HIP(this, launch_fill_double(ctx->grid->dT.p, G, 0.0, s));

@@ -213,6 +213,10 @@ StepParams eph_context::step_params() {
   return p;
 }
 
+void eph_context::drop_candidates() {
+  if (cstate.p) (void)launch_fill_int(cstate.p, 2, 0, stream);
+}
+
 WorkView eph_context::work_view() {
   WorkView w;
   w.nbr = nbr.p; w.nnbr = nnbr.p; w.act = act.p; w.nact = nact.p;
@@ -220,6 +224,17 @@ WorkView eph_context::work_view() {
   w.ghost_owner = ghost_owner.p; w.ghost_shift = ghost_shift.p;
   w.cA = colA.p; w.cB = colB.p;
   w.vw = colour == 3 ? colA.p : vel.p;
+  const double delta = cand_delta < 0.0 ? 0.1 * beta.r_cutoff : cand_delta;
+  if (delta > 0.0 && cstate.p && cand.p && cand.n >= act.n && ncand.n >= nact.n &&
+      cref.n >= static_cast<size_t>(stride)) {
+    w.cand = cand.p; w.ncand = ncand.p; w.cref = cref.p; w.cstate = cstate.p;
+    const double rc = beta.r_cutoff + delta;
+    w.rcand2 = rc * rc;
+    // the candidates are retaken once an atom has moved 0.45 delta (not
+    // delta/2): entries left out are then farther than rc + 0.1 delta, a
+    // margin far above the rounding of the distances
+    w.half_delta2 = (0.45 * delta) * (0.45 * delta);
+  }
   return w;
 }
 
@@ -253,6 +268,9 @@ int eph_context::reserve_atoms(int nl, int ng) {
   EPH_HIP(this, tag.reserve(nall));
   EPH_HIP(this, nnbr.reserve(st));
   EPH_HIP(this, nact.reserve(st));
+  EPH_HIP(this, ncand.reserve(st));
+  EPH_HIP(this, cref.reserve(st));
+  EPH_HIP(this, cstate.reserve(4));
   EPH_HIP(this, stage.reserve(3 * nall));
   EPH_HIP(this, xi_in.reserve(3 * (size_t)st));
   if (st != stride || f.n < 3 * (size_t)st) {
@@ -280,6 +298,7 @@ int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, con
   for (size_t i = 0; i < nall; ++i)
     if (ty[i] < 1 || ty[i] > ntypes) return fail(EPH_ERR_ARGS, "atom type out of range");
   int rc = reserve_atoms(nl, ng);
+  if (rc == EPH_OK) drop_candidates();
   if (rc != EPH_OK) return rc;
   if (species_dev.n < (size_t)ntypes) {
     EPH_HIP(this, species_dev.reserve(ntypes));
@@ -309,6 +328,7 @@ int eph_context::set_atoms(int nl, int ng, const double* x, const double* v, con
 
 int eph_context::set_ghost_map(int ng, const int* owner, const double* shift) {
   if (ng != nghost) return fail(EPH_ERR_ARGS, "ghost map size differs from nghost");
+  drop_candidates();
   if (ng == 0) return EPH_OK;
   for (int g = 0; g < ng; ++g)
     if (owner[g] < 0 || owner[g] >= nlocal) return fail(EPH_ERR_ARGS, "ghost owner out of range");
@@ -327,6 +347,7 @@ int eph_context::set_ghost_map(int ng, const int* owner, const double* shift) {
 int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neighmask) {
   if (!have_atoms) return fail(EPH_ERR_STATE, "atoms must be set before the neighbour list");
   if (nl != nlocal) return fail(EPH_ERR_ARGS, "neighbour list size differs from nlocal");
+  drop_candidates();
   if (!off) return fail(EPH_ERR_ARGS, "CSR offsets missing");
   if (off[0] != 0) return fail(EPH_ERR_ARGS, "CSR offsets must start at 0");
   if (off[nl] > 0 && !idx) return fail(EPH_ERR_ARGS, "CSR indices missing");
@@ -341,6 +362,7 @@ int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neigh
   maxnbr = mx;
   EPH_HIP(this, nbr.reserve((size_t)std::max(mx, 1) * stride));
   EPH_HIP(this, act.reserve((size_t)std::max(mx, 1) * stride));
+  EPH_HIP(this, cand.reserve((size_t)std::max(mx, 1) * stride));
   EPH_HIP(this, csr_off.reserve(nl + 1));
   EPH_HIP(this, csr_idx.reserve(std::max<long>(nnz, 1)));
   EPH_HIP(this, hipMemcpyAsync(csr_off.p, off, (nl + 1) * sizeof(int), hipMemcpyHostToDevice, stream));
@@ -798,6 +820,14 @@ int eph_set_masses(eph_context* ctx, const double* mass, int ntypes) {
 int eph_set_groupbit(eph_context* ctx, int groupbit) {
   if (!ctx) return EPH_ERR_ARGS;
   ctx->groupbit = groupbit;
+  ctx->drop_candidates();
+  return EPH_OK;
+}
+
+int eph_set_candidate_margin(eph_context* ctx, double delta) {
+  if (!ctx || !(delta == delta)) return EPH_ERR_ARGS;
+  ctx->cand_delta = delta;
+  ctx->drop_candidates();
   return EPH_OK;
 }
 

@@ -105,6 +105,13 @@ struct eph_context {
   eph::DevArray<double> f, fE, fR, xi_in, stage, mass, scal;  // scal: [0] E_local [1] Ee
   eph::DevArray<int> type, mask, tag, ghost_owner, nbr, nnbr, act, nact, csr_off, csr_idx,
       flags_dev, list_bad, species_dev;  // species_dev: element of each LAMMPS type
+  // candidate list of the cooperative density sweep (WorkView::cand);
+  // cand_delta < 0: 0.1 r_cutoff, 0: none
+  eph::DevArray<int> cand, ncand, cstate;
+  eph::DevArray<double4> cref;
+  double cand_delta = -1.0;
+  // after any change of the lists, ghosts, types, masks or group
+  void drop_candidates();
   int xi_mode_next = 0;  // 1 once eph_set_noise injected values
   // kernel forms of the two sweeps (-1 chosen by system size; see eph_autotune)
   int form_density = -1, form_force = -1;

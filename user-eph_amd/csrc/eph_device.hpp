@@ -111,6 +111,17 @@ struct WorkView {
   double4* cA = nullptr;
   double4* cB = nullptr;
   const double4* vw = nullptr;  // velocities the w sweep uses (vel, or zv)
+  // Candidate list of the cooperative density sweep: the entries of the full
+  // list with r^2 < rcand2 = (rc + delta)^2 at the positions cref.  While no
+  // local atom has moved delta/2 from cref, every entry outside it has r > rc,
+  // so the rho loop may walk the candidates instead of the full list (same
+  // active entries, same order).  rcand2 = 0: not used.
+  int* cand = nullptr;          // ELL, same stride and width as nbr
+  int* ncand = nullptr;
+  double4* cref = nullptr;      // [nlocal]
+  int* cstate = nullptr;        // [0]: candidates valid, [1]: an atom moved >= delta/2
+  double rcand2 = 0.0;
+  double half_delta2 = 0.0;     // (delta/2)^2
 };
 
 // Ordered deposit of per-atom values into grid cells (bit-identical to a

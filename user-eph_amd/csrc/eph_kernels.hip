@@ -266,10 +266,17 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   const int lane = threadIdx.x & 63;
   const double4 pi = a.pos[i];
   const bool grp = valid && (a.mask[i] & a.groupbit) != 0;
-  const int nn = grp ? w.nnbr[i] : 0;
+  // the rho loop walks the candidate list when it is valid and no atom has
+  // moved delta/2 since it was taken; otherwise the full list, taking the
+  // candidates on the way (when a candidate list is kept at all)
+  const bool keep = w.rcand2 > 0.0;
+  const bool use_cand = keep && w.cstate[0] != 0 && w.cstate[1] == 0;
+  const bool take = keep && !use_cand;
+  const int* __restrict__ list = use_cand ? w.cand : w.nbr;
+  const int nn = grp ? (use_cand ? w.ncand[i] : w.nnbr[i]) : 0;
   const int nnmax = wave_max(nn);
   double rho = 0.0;
-  int na = 0;
+  int na = 0, nc = 0;
   {
     const double2* __restrict__ pos2 = reinterpret_cast<const double2*>(a.pos);
     // kCoopRho entries per iteration: the positions of entry k + e land at
@@ -278,9 +285,9 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
     constexpr int U = kCoopRho;
     int jj[U], nx[U];
 #pragma unroll
-    for (int e = 0; e < U; ++e) jj[e] = e < nn ? w.nbr[e * stride + i] : 0;
+    for (int e = 0; e < U; ++e) jj[e] = e < nn ? list[e * stride + i] : 0;
 #pragma unroll
-    for (int e = 0; e < U; ++e) nx[e] = U + e < nn ? w.nbr[(U + e) * stride + i] : 0;
+    for (int e = 0; e < U; ++e) nx[e] = U + e < nn ? list[(U + e) * stride + i] : 0;
     if (nnmax > 0) {
 #pragma unroll
       for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, jj[e], lane, stage + 128 * e);
@@ -302,7 +309,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
 #pragma unroll
         for (int e = 0; e < U; ++e) {
           jj[e] = nx[e];
-          nx[e] = k + 2 * U + e < nn ? w.nbr[(k + 2 * U + e) * stride + i] : 0;
+          nx[e] = k + 2 * U + e < nn ? list[(k + 2 * U + e) * stride + i] : 0;
         }
       }
 #pragma unroll
@@ -316,11 +323,19 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
             w.act[na * stride + i] = jc[e];
             ++na;
           }
+          if (take && r2 < w.rcand2) {
+            w.cand[nc * stride + i] = jc[e];
+            ++nc;
+          }
         }
       }
     }
   }
   if (valid) w.nact[i] = na;
+  if (take && valid) {
+    w.ncand[i] = nc;
+    w.cref[i] = pi;
+  }
   const double alpha = density_alpha<MULTI>(grp, rho, pi, t, w);
   double wx = 0.0, wy = 0.0, wz = 0.0;
   if (FRIC) {
@@ -358,6 +373,28 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
     }
   }
   if (valid) density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
+}
+
+// Before the sweep: did any local atom move delta/2 or more since the
+// candidates were taken (a non-finite distance counts as moved)?
+__global__ __launch_bounds__(kBlock) void k_cand_check(int nlocal, const double4* __restrict__ pos,
+                                                       WorkView w) {
+  if (w.cstate[0] == 0) return;
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  bool moved = false;
+  if (i < nlocal) {
+    const double4 x = pos[i], r = w.cref[i];
+    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+    moved = !(dx * dx + dy * dy + dz * dz < w.half_delta2);
+  }
+  if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(w.cstate + 1, 1);
+}
+
+// After the sweep: a sweep that walked the full list has taken the candidates
+__global__ void k_cand_flags(int* cstate) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  if (cstate[0] == 0 || cstate[1] != 0) cstate[0] = 1;
+  cstate[1] = 0;
 }
 
 // 256 atoms per block, tables through the caches, cooperative gathers
@@ -1718,7 +1755,11 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
     return hipGetLastError();
   }
   const int g = nblk(a.nlocal);
-#define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, w)
+  const bool keep = w.rcand2 > 0.0 && w.cstate;
+  WorkView wc = w;
+  if (!keep) wc.rcand2 = 0.0;
+  if (keep) k_cand_check<<<g, kBlock, 0, s>>>(a.nlocal, a.pos, wc);
+#define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, wc)
 #define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
 #define EPH_DW_F(XI) if (fric) { EPH_DW_M(XI, true); } else { EPH_DW_M(XI, false); }
   switch (p.xi_mode) {
@@ -1729,6 +1770,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
 #undef EPH_DW_F
 #undef EPH_DW_M
 #undef EPH_DW
+  if (keep) k_cand_flags<<<1, 64, 0, s>>>(w.cstate);
   return hipGetLastError();
 }
 
@@ -1999,9 +2041,11 @@ bool batch_lds_possible(const TablesView& t) {
 }
 
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
-                                  const WorkView& w, const Chunk* chunks, int nchunks,
+                                  const WorkView& w_in, const Chunk* chunks, int nchunks,
                                   const uint32_t* seeds, const uint64_t* step, hipStream_t s,
                                   int variant) {
+  WorkView w = w_in;
+  w.rcand2 = 0.0;  // replicas walk their full lists
   if (nchunks == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
   if (variant == 1 && batch_lds_possible(t)) {

@@ -166,6 +166,8 @@ int eph_md::upload(eph_context* ctx) {
   if ((rc = ctx->reserve_atoms(nlocal, ghost_cap))) return fail(rc, ctx->err);
   EPH_HIP(this, ctx->nbr.reserve((size_t)cap_f * ctx->stride));
   EPH_HIP(this, ctx->act.reserve((size_t)cap_f * ctx->stride));
+  EPH_HIP(this, ctx->cand.reserve((size_t)cap_f * ctx->stride));
+  ctx->drop_candidates();
   EPH_HIP(this, ctx->ghost_owner.reserve(ghost_cap));
   EPH_HIP(this, ctx->ghost_shift.reserve(ghost_cap));
   if ((rc = ctx->set_atoms(nlocal, nghost, x.data(), v.data(), type.data(), mask.data(), tag.data())))
@@ -232,6 +234,7 @@ int eph_md::upload(eph_context* ctx) {
   w.lists.nbr_l = lj_nbr.p; w.lists.nn_l = lj_nn.p; w.lists.cap_l = cap_l; w.lists.cl2 = cl * cl;
   w.lists.stride = ctx->stride;
   w.go = flags.p; w.overflow = flags.p + 1; w.nrebuild = flags.p + 2; w.nghost = flags.p + 4;
+  w.cand_valid = ctx->cstate.p;
   w.cnt = cnt.p; w.start = start.p; w.bin_of = bin_of.p; w.bcount = bcount.p; w.bstart = bstart.p;
   w.bfill = bfill.p; w.items = items.p; w.scan_tmp = scan_tmp.p; w.xref = xref.p;
   EPH_HIP(this, hipStreamSynchronize(s));

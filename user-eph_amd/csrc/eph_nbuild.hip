@@ -227,10 +227,12 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
 }
 
 __global__ __launch_bounds__(kBlock) void k_nb_finish(int nlocal, const double4* __restrict__ pos,
-                                                      double4* ref, int* go, int* nreb) {
+                                                      double4* ref, int* go, int* nreb,
+                                                      int* cand_valid) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (!*go) return;
   if (i < nlocal) ref[i] = pos[i];
+  if (i == 0 && cand_valid) *cand_valid = 0;  // new lists: candidates are retaken
   // the last block clears the flag only after every block has read it: the
   // flag is cleared by the next step's k_nb_reset instead
   if (i == 0) atomicAdd(nreb, 1);
@@ -366,7 +368,8 @@ hipError_t launch_nb_build(const NbBuild& w, hipStream_t s) {
   k_nb_bin_sort<<<nbk(w.nbins), kBlock, 0, s>>>(w.nbins, w.bstart, w.items, go);
   k_nb_lists<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.at.rep, w.box, w.bstart,
                                               w.items, w.lists, w.overflow, go);
-  k_nb_finish<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.xref, w.go, w.nrebuild);
+  k_nb_finish<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.xref, w.go, w.nrebuild,
+                                                w.cand_valid);
   return hipGetLastError();
 }
 

@@ -43,6 +43,7 @@ struct NbBuild {
   int* nghost = nullptr;    // [4] ghosts after each pass; [3] is the result
   int* overflow = nullptr;  // [1] bit 0 ghosts, bit 1 lists
   int* nrebuild = nullptr;  // [1] count of device rebuilds
+  int* cand_valid = nullptr;  // [1] cleared by a rebuild (the density sweep's candidates)
   int* cnt = nullptr;       // [nlocal + ghost_cap]
   int* start = nullptr;     // [nlocal + ghost_cap + 1]
   int* bin_of = nullptr;    // [nlocal + ghost_cap]

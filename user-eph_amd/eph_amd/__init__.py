@@ -53,6 +53,7 @@ _SIG = {
     "eph_reset_dt": (C.c_int, [vp, C.c_double]),
     "eph_set_masses": (C.c_int, [vp, dp, C.c_int]),
     "eph_set_groupbit": (C.c_int, [vp, C.c_int]),
+    "eph_set_candidate_margin": (C.c_int, [vp, C.c_double]),
     "eph_get_params": (C.c_int, [vp, ip, ip, ip, dp, dp, dp, dp]),
     "eph_set_atoms": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, ip, ip, ip]),
     "eph_set_ghost_map": (C.c_int, [vp, C.c_int, ip, dp]),
@@ -288,6 +289,10 @@ class FixEPH:
         self._check(lib().eph_set_masses(self.h, _d(m), self.ntypes))
 
     def set_groupbit(self, gb): self._check(lib().eph_set_groupbit(self.h, gb))
+
+    def set_candidate_margin(self, delta):
+        """delta < 0: 0.1 r_cutoff (default); 0: the rho loop always scans the full list"""
+        self._check(lib().eph_set_candidate_margin(self.h, C.c_double(delta)))
 
     def set_atoms(self, s):
         """s: object with nlocal, nghost, x, v ([nall][3]), type, mask, tag"""
