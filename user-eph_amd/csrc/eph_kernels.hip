@@ -4,7 +4,9 @@
 // expressions in the reference's operation order, one thread per atom (or
 // per grid cell) summing its neighbours in list order, with FMA contraction
 // disabled at build time.  Results are therefore bit-identical to the serial
-// CPU FixEPH on the same inputs; no atomics touch floating-point sums.
+// CPU FixEPH on the same inputs; no atomics touch floating-point sums.  In
+// the cooperative forms a wavefront fetches its atoms' neighbour records
+// together through LDS; the sums themselves stay one thread per atom.
 #include "eph_devfn.hpp"
 #include "eph_rng.hpp"
 
