@@ -59,31 +59,64 @@ def write_inputs(tmp, cells, a, grid, species):
     return beta, names, tin
 
 
-def cpu_baseline(args_fix, species, snap, md, dt, nsteps, threads, first=1):
+def host_cpu_info():
+    """CPU model (lscpu / /proc/cpuinfo) and online core count of this host."""
+    import subprocess
+    model = None
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except (OSError, subprocess.SubprocessError):
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    return {"model": model, "nproc": os.cpu_count()}
+
+
+def cpu_baseline(args_fix, species, snap, md, dt, nsteps, threads, runs=3, first=1):
     """The oracle (C++ restatement of FixEPH) on the same system; its per-atom loops
     run on `threads` host threads (OpenMP), the deposit and the FDM solve serially
-    as in the reference."""
+    as in the reference.  One untimed warm-up step, then `runs` timed runs of
+    `nsteps` steps each from the same starting state; returns the median rate."""
     import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import eph_oracle as eo
     from eph_amd import inputs
     used = eo.set_threads(threads)
-    fix = eo.Fix(args_fix, species, snap.boxlo, snap.boxhi, dt, inputs.BOLTZ, inputs.FTM2V,
-                 masses=[inputs.MASS_CU] * species)
-    md.lj(snap)
-    t_fix = 0.0
-    for s in range(first, first + nsteps):
-        t0 = time.perf_counter()
-        fix.initial_integrate(snap)
-        t_fix += time.perf_counter() - t0
-        md.update_ghosts(snap)
-        t0 = time.perf_counter()
-        fix.post_force(snap, step=s)
-        fix.final_integrate(snap)
-        fix.end_of_step(snap, s)
-        t_fix += time.perf_counter() - t0
-        log(f"cpu baseline ({used} threads) step {s - first + 1}/{nsteps}: {t_fix:.1f} s")
-    return snap.nlocal * nsteps / t_fix, t_fix, used
+
+    def one_run(s0, nst, tag):
+        sn = copy_snapshot(s0)
+        fix = eo.Fix(args_fix, species, sn.boxlo, sn.boxhi, dt, inputs.BOLTZ, inputs.FTM2V,
+                     masses=[inputs.MASS_CU] * species)
+        md.lj(sn)
+        t_fix = 0.0
+        for s in range(first, first + nst):
+            t0 = time.perf_counter()
+            fix.initial_integrate(sn)
+            t_fix += time.perf_counter() - t0
+            md.update_ghosts(sn)
+            t0 = time.perf_counter()
+            fix.post_force(sn, step=s)
+            fix.final_integrate(sn)
+            fix.end_of_step(sn, s)
+            t_fix += time.perf_counter() - t0
+        log(f"cpu baseline ({used} threads) {tag}: {nst} steps in {t_fix:.2f} s")
+        return t_fix
+
+    one_run(snap, 1, "warm-up")
+    times = [one_run(snap, nsteps, f"run {r + 1}/{runs}") for r in range(runs)]
+    t_med = float(np.median(times))
+    return snap.nlocal * nsteps / t_med, t_med, used, times
 
 
 def host_path_rate(fargs, species, snap, inf, dt, nsteps):
@@ -288,8 +321,10 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--cells", type=int, default=80)
-    ap.add_argument("--cpu-steps", type=int, default=2, help="one-thread CPU baseline steps")
-    ap.add_argument("--cpu-all-steps", type=int, default=6, help="all-thread CPU baseline steps")
+    ap.add_argument("--cpu-steps", type=int, default=1,
+                    help="one-thread CPU baseline steps per timed run (median of 3 runs)")
+    ap.add_argument("--cpu-all-steps", type=int, default=2,
+                    help="all-thread CPU baseline steps per timed run (median of 3 runs)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--host-steps", type=int, default=3,
                     help="steps of the host-array (PCIe-inclusive) rate, 0 to skip")
@@ -340,10 +375,12 @@ def main():
         # timed before the device run, while the harness lists still match snap0
         log("cpu baseline ...")
         snap0 = copy_snapshot(snap_host) if snap_host is not None else md.snapshot()
-        threads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or
-                             (os.cpu_count() or 1)))
+        # every core this process may use: the host's online cores, bounded by
+        # the OMP_NUM_THREADS share the machine gives one GPU when it is set
+        share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+        threads = max(1, min(share, os.cpu_count() or 1) if share > 0 else (os.cpu_count() or 1))
         if args.cpu_steps > 0:
-            cpu1 = cpu_baseline(fargs, species, copy_snapshot(snap0), md, dt, args.cpu_steps, 1)
+            cpu1 = cpu_baseline(fargs, species, snap0, md, dt, args.cpu_steps, 1)
         if args.cpu_all_steps > 0:
             cpu = cpu_baseline(fargs, species, snap0, md, dt, args.cpu_all_steps, threads)
     md.attach(fix)
@@ -392,8 +429,18 @@ def main():
     nlist = fix.get("nlist").mean()
     fix_s = res["fix_ms"] / 1e3
     value = N * args.steps / fix_s
-    # the two neighbour sweeps, each against its algorithmic bytes per atom;
-    # the roofline line reports the one with the longer average launch
+    # The two neighbour sweeps against their algorithmic bytes per atom-step
+    # (FP64 SoA, int32 indices, K = full-list entries per atom, each per-atom
+    # array read or written once, gathered neighbour data and tables cached):
+    #   density sweep = calculate_environment fused with force_prl stage 1:
+    #     x 24 + type 4 + count 4 + list 4K + v 24 + mask 4 + rho 8 + w 24 = 92 + 4K
+    #   force sweep = force_prl stages 2 + 3 fused:
+    #     x 24 + type 4 + count 4 + list 4K + rho 8 + w 24 + xi 24 + T_e 8
+    #     + f in/out 48 + f_EPH 24 + f_RNG 24 = 156 + 4K (the K entries are
+    #     the list the reference loops walk; the active list is this design's)
+    #   force_prl as a whole (both sweeps) = 248 + 8K.
+    # design_bytes_per_atom is what these kernels move by their own layout
+    # (active-list write and read, the noise and alpha record slots, tags).
     prl_ms, prl_n = rep.get("prl_forces", (float("nan"), 1))
     dw_ms, dw_n = rep.get("density_w", (float("nan"), 1))
     # FP64 operations of the reference's expressions (+ - * / each one):
@@ -402,17 +449,31 @@ def main():
     # spline (two with several species) + 25 friction + 25 random
     fpair = 3 + 5 + 6 * (2 if species > 1 else 1) + 50
     sweeps = {
-        "k_prl_forces": {"bytes_per_atom": 200.0 + 4.0 * nact, "flops_per_atom": fpair * nact,
-                         "avg_launch_ms": prl_ms / max(prl_n, 1)},
-        "k_density_w": {"bytes_per_atom": 132.0 + 4.0 * nlist + 4.0 * nact,
+        "k_density_w": {"bytes_per_atom": 92.0 + 4.0 * nlist,
+                        "design_bytes_per_atom": 132.0 + 4.0 * nlist + 4.0 * nact,
                         "flops_per_atom": 8.0 * nlist + 43.0 * nact,
                         "avg_launch_ms": dw_ms / max(dw_n, 1)},
+        "k_prl_forces": {"bytes_per_atom": 156.0 + 4.0 * nlist,
+                         "design_bytes_per_atom": 200.0 + 4.0 * nact,
+                         "flops_per_atom": fpair * nact,
+                         "avg_launch_ms": prl_ms / max(prl_n, 1)},
     }
     for v in sweeps.values():
         v["achieved_GBs"] = v["bytes_per_atom"] * N / (v["avg_launch_ms"] * 1e-3) / 1e9
         v["frac"] = v["achieved_GBs"] / HBM_PEAK_GBS
+        v["design_GBs"] = v["design_bytes_per_atom"] * N / (v["avg_launch_ms"] * 1e-3) / 1e9
         v["fp64_TFLOPs"] = v["flops_per_atom"] * N / (v["avg_launch_ms"] * 1e-3) / 1e12
         v["fp64_frac"] = v["fp64_TFLOPs"] / FP64_PEAK_TFLOPS
+    both_ms = sweeps["k_density_w"]["avg_launch_ms"] + sweeps["k_prl_forces"]["avg_launch_ms"]
+    prl_bytes = 248.0 + 8.0 * nlist
+    force_prl = {
+        "kernels": ["k_density_w", "k_prl_forces"],
+        "bytes_per_atom": prl_bytes, "ms": both_ms,
+        "achieved_GBs": prl_bytes * N / (both_ms * 1e-3) / 1e9,
+        "target_frac": 0.6,
+        "atom_steps_per_s": N / (both_ms * 1e-3),
+    }
+    force_prl["frac"] = force_prl["achieved_GBs"] / HBM_PEAK_GBS
     dom = max(sweeps, key=lambda k: sweeps[k]["avg_launch_ms"])
     achieved = sweeps[dom]["achieved_GBs"]
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rep.items()}
@@ -436,7 +497,7 @@ def main():
             "Te_grid": [grid] * 3, "dt_ps": dt, "flags": 7, "model": 4,
             "md_ms_per_step": res["total_ms"] / args.steps,
             "pair_ms_per_step": res["pair_ms"] / args.steps,
-            "host_rebuilds": res["rebuilds"], "host_rebuild_ms": res["rebuild_ms"],
+            "list_rebuilds": res["rebuilds"], "list_check_and_rebuild_ms": res["rebuild_ms"],
             "fdm_blowup_step": blowup_step,
             "kernels": kernels,
             "sweep_forms": tuned,
@@ -450,7 +511,9 @@ def main():
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "bytes_per_atom": sweeps[dom]["bytes_per_atom"],
+            "design_bytes_per_atom": sweeps[dom]["design_bytes_per_atom"],
             "avg_launch_ms": sweeps[dom]["avg_launch_ms"],
+            "force_prl": force_prl,
             "fp64": {"achieved": sweeps[dom]["fp64_TFLOPs"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": sweeps[dom]["fp64_frac"],
                      "flops_per_atom": sweeps[dom]["flops_per_atom"]},
@@ -458,16 +521,23 @@ def main():
         },
         "cpu_baseline": None,
     }
+    hostcpu = host_cpu_info() if (cpu is not None or cpu1 is not None) else None
     if cpu is not None:
-        v_cpu, t_cpu, used = cpu
+        v_cpu, t_cpu, used, times = cpu
         out["cpu_baseline"] = {"value": v_cpu, "unit": "atom-steps/s", "cores": used, "kind": "port",
-                               "sample": f"{args.cpu_all_steps} fix steps of the same {N}-atom "
-                                         f"system ({t_cpu:.1f} s), the oracle's per-atom loops on "
+                               "host_cpu": hostcpu,
+                               "run_seconds": times,
+                               "sample": f"median of 3 runs of {args.cpu_all_steps} fix steps of the "
+                                         f"same {N}-atom system after one warm-up step "
+                                         f"({t_cpu:.2f} s per run); the oracle's per-atom loops on "
                                          f"{used} host threads, deposit and FDM serial"}
     if cpu1 is not None:
-        v1, t1, _ = cpu1
+        v1, t1, _, times1 = cpu1
         out["cpu_baseline_1core"] = {"value": v1, "unit": "atom-steps/s", "cores": 1, "kind": "port",
-                                     "sample": f"{args.cpu_steps} fix steps ({t1:.1f} s), one thread"}
+                                     "host_cpu": hostcpu, "run_seconds": times1,
+                                     "sample": f"median of 3 runs of {args.cpu_steps} fix steps "
+                                               f"after one warm-up step ({t1:.2f} s per run), "
+                                               f"one thread"}
     if args.calibrate_copy > 0:
         # plain copy kernel (k_copy_probe) of a known size, in this same process
         nbytes = args.calibrate_copy * (1 << 20)

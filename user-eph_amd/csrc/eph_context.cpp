@@ -372,12 +372,15 @@ int eph_context::set_neighbors(int nl, const int* off, const int* idx, int neigh
   // the indices are range-checked on the device while they are converted
   EPH_HIP(this, list_bad.reserve(1));
   EPH_HIP(this, hipMemsetAsync(list_bad.p, 0, sizeof(int), stream));
+  // rows are bounded by the width the host offsets give, so a device copy of
+  // the offsets that differs from them cannot steer writes past the buffers
   EPH_HIP(this, launch_csr_to_ell(nl, csr_off.p, csr_idx.p, neighmask, nall, nbr.p, nnbr.p, stride,
-                                  list_bad.p, stream));
+                                  std::max(mx, 1), list_bad.p, stream));
   int bad = 0;
   EPH_HIP(this, hipMemcpyAsync(&bad, list_bad.p, sizeof(int), hipMemcpyDeviceToHost, stream));
   EPH_HIP(this, hipStreamSynchronize(stream));
   have_list = bad == 0;
+  if (bad & 2) return fail(EPH_ERR_STATE, "neighbour offsets changed on their way to the device (the list is dropped)");
   if (bad) return fail(EPH_ERR_ARGS, "neighbour index out of range (the list is dropped)");
   return EPH_OK;
 }

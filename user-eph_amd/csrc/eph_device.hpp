@@ -160,9 +160,10 @@ struct Chunk {
 
 // ---- launch wrappers (eph_kernels.hip) ----
 hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* shift, hipStream_t s);
-// indices outside [0, nall) set *bad (and are stored as 0)
+// indices outside [0, nall) set bit 0 of *bad (and are stored as 0); a row
+// with more than `width` entries sets bit 1 and is stored empty
 hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int nall,
-                             int* ell, int* nn, int stride, int* bad, hipStream_t s);
+                             int* ell, int* nn, int stride, int width, int* bad, hipStream_t s);
 // form: 0 one block per 256 atoms with tables read from global memory;
 // kLdsForm1024/512/256 persistent blocks of that many threads with the
 // tables in LDS (when they fit); -1 by size (LDS with 1024 threads for the

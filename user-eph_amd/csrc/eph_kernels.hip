@@ -30,10 +30,19 @@ __global__ __launch_bounds__(kBlock) void k_ghost_xv(AtomsView a, const int* __r
 __global__ __launch_bounds__(kBlock) void k_csr_to_ell(int nlocal, const int* __restrict__ off,
                                                        const int* __restrict__ idx, int neighmask,
                                                        int nall, int* __restrict__ ell,
-                                                       int* __restrict__ nn, int stride, int* bad) {
+                                                       int* __restrict__ nn, int stride, int width,
+                                                       int* bad) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nlocal) return;
-  const int b = off[i], e = off[i + 1];
+  const int b = off[i];
+  int e = off[i + 1];
+  // a row longer than the ELL width (or with decreasing offsets) is refused
+  // and left empty: the device copy of the offsets is never trusted to stay
+  // inside the buffer
+  if (e - b > width || e < b) {
+    atomicOr(bad, 2);
+    e = b;
+  }
   for (int k = b; k < e; ++k) {
     int j = idx[k] & neighmask;
     if (j < 0 || j >= nall) {
@@ -1666,10 +1675,10 @@ hipError_t launch_ghost_xv(const AtomsView& a, const int* owner, const double4* 
 }
 
 hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int neighmask, int nall,
-                             int* ell, int* nn, int stride, int* bad, hipStream_t s) {
+                             int* ell, int* nn, int stride, int width, int* bad, hipStream_t s) {
   if (nlocal == 0) return hipSuccess;
   k_csr_to_ell<<<nblk(nlocal), kBlock, 0, s>>>(nlocal, off, idx, neighmask, nall, ell, nn, stride,
-                                               bad);
+                                               width, bad);
   return hipGetLastError();
 }
 

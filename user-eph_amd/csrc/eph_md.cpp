@@ -191,8 +191,9 @@ int eph_md::upload(eph_context* ctx) {
   if (!lj.idx.empty())
     EPH_HIP(this, hipMemcpyAsync(lj_idx.p, lj.idx.data(), lj.idx.size() * sizeof(int), hipMemcpyHostToDevice, s));
   EPH_HIP(this, ctx->list_bad.reserve(1));
+  EPH_HIP(this, launch_fill_int(ctx->list_bad.p, 1, 0, s));
   EPH_HIP(this, launch_csr_to_ell(nlocal, lj_off.p, lj_idx.p, ~0, nlocal + nghost, lj_nbr.p, lj_nn.p,
-                                  lj_stride, ctx->list_bad.p, s));
+                                  lj_stride, std::max(mx, 1), ctx->list_bad.p, s));
   EPH_HIP(this, xref.reserve(nlocal));
   EPH_HIP(this, hipMemcpyAsync(xref.p, ctx->pos.p, nlocal * sizeof(double4), hipMemcpyDeviceToDevice, s));
   EPH_HIP(this, partial.reserve(1024));
@@ -237,7 +238,9 @@ int eph_md::upload(eph_context* ctx) {
   w.cand_valid = ctx->cstate.p;
   w.cnt = cnt.p; w.start = start.p; w.bin_of = bin_of.p; w.bcount = bcount.p; w.bstart = bstart.p;
   w.bfill = bfill.p; w.items = items.p; w.scan_tmp = scan_tmp.p; w.xref = xref.p;
+  EPH_HIP(this, hipMemcpyAsync(pinned, ctx->list_bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
   EPH_HIP(this, hipStreamSynchronize(s));
+  if (pinned[0]) return fail(EPH_ERR_STATE, "pair list refused on the device (index or row width)");
   return EPH_OK;
 }
 
