@@ -277,6 +277,19 @@ int eph_grid_T_mean(eph_grid* g, double* out);        /* eph_fdm.h:189-196 */
 int eph_grid_save_state(eph_grid* g, const char* path);                  /* :226-265 */
 int eph_grid_save_temperature(eph_grid* g, const char* base, int n);     /* :198-224 */
 
+/* No counterpart in the reference.  eph_grid_insert_energy for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc): x, y, z, E are n values each in
+ * device memory; the deposit's scratch is one fixed array per GPU of
+ * eph_noalloc_scratch_bytes() bytes (20 MiB), enough for 16 n + 20 G bytes
+ * plus about 2 KB (G = grid cells); larger inputs return EPH_ERR_ARGS.  The
+ * work is enqueued on the grid's stream and the call returns without waiting
+ * (eph_grid_get and eph_grid_solve order after it).  All calls on one GPU
+ * share that scratch: calls that overlap in time on one GPU race. */
+int eph_grid_insert_energy_noalloc(eph_grid* g, int n, const double* x, const double* y,
+                                   const double* z, const double* E);
+long long eph_noalloc_scratch_bytes(void);
+
 #ifdef __cplusplus
 }
 #endif

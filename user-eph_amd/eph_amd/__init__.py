@@ -102,6 +102,8 @@ _SIG = {
     "eph_grid_set_cell": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_double, C.c_double,
                                     C.c_double, C.c_double, C.c_double, C.c_int, C.c_int]),
     "eph_grid_insert_energy": (C.c_int, [vp, C.c_int, dp, dp, dp, dp]),
+    "eph_grid_insert_energy_noalloc": (C.c_int, [vp, C.c_int, vp, vp, vp, vp]),
+    "eph_noalloc_scratch_bytes": (C.c_longlong, []),
     "eph_grid_solve": (C.c_int, [vp, ip]),
     "eph_grid_get": (C.c_int, [vp, C.c_int, dp]),
     "eph_grid_set": (C.c_int, [vp, C.c_int, dp]),
@@ -484,6 +486,11 @@ class Grid:
     def insert_energy(self, x, y, z, E):
         x, y, z, E = (f64(np.atleast_1d(a)) for a in (x, y, z, E))
         self._check(lib().eph_grid_insert_energy(self.h, len(x), _d(x), _d(y), _d(z), _d(E)))
+
+    def insert_energy_noalloc(self, n, x, y, z, E):
+        """insert_energy for n points already in device memory (x, y, z, E are
+        device addresses); allocates nothing, returns once the work is queued"""
+        self._check(lib().eph_grid_insert_energy_noalloc(self.h, int(n), x, y, z, E))
 
     def solve(self):
         n = C.c_int()
