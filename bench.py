@@ -153,20 +153,7 @@ def host_path_rate(fargs, species, snap, inf, dt, nsteps):
     return snap.nlocal * nsteps / t_fix, t_fix, t_reneigh
 
 
-ENS_REPLICAS, ENS_STEPS, ENS_DT = 64, 300, 2.5e-4
-
-
-def write_uniform_grid(path, n, L, steps):
-    import numpy as np
-    idx = np.indices((n, n, n))[::-1].reshape(3, -1).T
-    with open(path, "w") as f:
-        f.write("# T_e grid: uniform 300 K\n#\n#\n")
-        f.write(f"{n} {n} {n} {steps}\n")
-        for _ in range(3):
-            f.write(f"0.0 {L!r}\n")
-        f.write("NULL\n")
-        f.write("\n".join(f"{i} {j} {k} 300.0 0.0 1.0 3e-06 0.1 1 0" for i, j, k in idx.tolist()))
-        f.write("\n")
+ENS_REPLICAS, ENS_STEPS = 64, 300
 
 
 def copy_snapshot(s):
@@ -178,22 +165,10 @@ def ensemble_bench(tmp, nrep, nsteps, warmup, sequential):
     """Batched ensemble: nrep replicas of n^3 fcc cells, n ~ logU[10, 25]
     (seeded), one 1 keV PKA each, own seed and n^3 T_e grid (19 sub-steps),
     advanced together; against the same replicas run one after another."""
-    import numpy as np
     import eph_amd as ea
     from eph_amd import inputs
-    a = 3.615
-    rng = np.random.default_rng(3)
-    ns = np.rint(np.exp(rng.uniform(np.log(10.0), np.log(25.0), nrep))).astype(int)
-    beta = os.path.join(tmp, "ens.beta")
-    names = inputs.workload_beta(beta, 1)
-    reps = []
-    for r, n in enumerate(ns.tolist()):
-        tin = os.path.join(tmp, f"ens{r}.T")
-        write_uniform_grid(tin, n, n * a, 19)
-        fargs = inputs.fix_args(beta, names, seed=7000 + r, flags="7", grid=(n, n, n),
-                                T_infile=tin, C_e=3e-6, kappa=0.1)
-        kw = dict(cells=n, a=a, ntypes=1, seed=900 + r, T=300.0, pka_eV=1000.0)
-        reps.append((kw, fargs))
+    ENS_DT = inputs.ENSEMBLE_DT
+    reps, _ = inputs.ensemble_replicas(tmp, nrep)
     t0 = time.time()
     ens = ea.Ensemble()
     for kw, fargs in reps:

@@ -162,7 +162,10 @@ int eph_set_atomic_energy(eph_context* ctx, const double* E);
  * which 0 rho [nall], 1 w [nall][3], 2 xi [nall][3], 3 f_EPH [nlocal][3],
  * 4 f_RNG [nlocal][3], 5 alpha [nall], 6 active neighbours (r < r_cutoff)
  * per local atom [nlocal], 7 list entries per local atom [nlocal];
- * eph/atomic also 8 rho_a [nall], 9 E [nall], 10 dE [nlocal], 11 T [nall] */
+ * eph/atomic also 8 rho_a [nall], 9 E [nall], 10 dE [nlocal], 11 T [nall];
+ * 12: 4 values about the density sweep's candidate list at its last run:
+ * waves that scanned the full list, waves in all, atoms found moved, 1 if the
+ * candidates of all atoms were retaken */
 int eph_get_state(eph_context* ctx, int which, double* out);
 /* energy deposited in the last end_of_step (E_local of fix_eph.cpp:357-402) */
 int eph_last_energy(eph_context* ctx, double* E_local);

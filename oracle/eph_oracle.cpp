@@ -303,7 +303,20 @@ struct OFDM {
     fd >> steps;
     double bx[6]; for (double& v : bx) fd >> v;
     set_box(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
-    fd >> param_file;
+    // :70 reads the parameter-file name here.  Grid files of the older
+    // layout (Examples/FDM/T.in) go straight to nine-column cell lines
+    // "i j k T S rho C kappa flag"; they are recognised by that first line
+    // and read with T_dyn = 0 and no parameter file.
+    std::string rest, next;
+    std::getline(fd, rest);  // the tokens may also follow the box on its line
+    next = rest;
+    while (fd && next.find_first_not_of(" \t\r") == std::string::npos) std::getline(fd, next);
+    std::vector<std::string> tok;
+    { std::istringstream ls(next); for (std::string t; ls >> t;) tok.push_back(t); }
+    const bool legacy = tok.size() == 9;
+    if (!legacy && tok.size() != 1) { err = "unrecognised line after the box"; return false; }
+    param_file = legacy ? "NULL" : tok[0];
+    std::istringstream first(legacy ? next : std::string());
     if (param_file != "NULL") {
       std::ifstream pf(param_file);
       if (!pf) { err = "cannot open parameter file " + param_file; return false; }
@@ -318,12 +331,15 @@ struct OFDM {
       E_e_T = OLinear(dT, Ce);
     }
     for (size_t i = 0; i < ntotal; ++i) {
-      int lx, ly, lz; fd >> lx >> ly >> lz;
+      std::istream& src = (legacy && i == 0) ? static_cast<std::istream&>(first) : fd;
+      int lx, ly, lz; src >> lx >> ly >> lz;
+      if (!src || lx < 0 || ly < 0 || lz < 0 || (size_t)lx >= nx || (size_t)ly >= ny ||
+          (size_t)lz >= nz) { err = "bad grid line"; return false; }
       size_t idx = lx + ly * nx + lz * nx * ny;
-      if (!fd || idx >= ntotal) { err = "bad grid line"; return false; }
-      fd >> T_e[idx] >> S_e[idx] >> rho_e[idx] >> C_e[idx] >> kappa_e[idx] >> flag[idx] >> T_dyn[idx];
+      src >> T_e[idx] >> S_e[idx] >> rho_e[idx] >> C_e[idx] >> kappa_e[idx] >> flag[idx];
+      if (legacy) T_dyn[idx] = 0; else src >> T_dyn[idx];
+      if (!src) { err = "truncated grid file"; return false; }
     }
-    if (!fd) { err = "truncated grid file"; return false; }
     return true;
   }
   size_t get_index(double x, double y, double z) const {  // :494-509

@@ -17,6 +17,12 @@
 //        runs nsteps of solve() (energy lines "x y z E" inserted before every
 //        solve), printing every cell's T (cell-centre get_T) after each
 //        multiple of <every>.
+//   fdm_save <gridfile> <dt> <nsteps> <efile|NULL> <state> <heat> <n> <nmore>
+//        the same nsteps, then save_state(<state>) and
+//        save_temperature(<heat>, <n>) (the files fix eph writes in post_run
+//        and every T_freq steps); then a new grid read from <state> runs
+//        <nmore> further solves with the same energies and its T is printed
+//        as above (the resume from T.restart).
 
 #include <fstream>
 #include <mpi.h>
@@ -144,6 +150,47 @@ int main(int argc, char** argv) {
                 std::printf("%a\n", g.get_T(x0 + (i + 0.5) * dx, y0 + (j + 0.5) * dy, z0 + (k + 0.5) * dz));
         }
       }
+    }
+    MPI_Finalize();
+    return 0;
+  }
+  if (cmd == "fdm_save" && argc == 10) {
+    MPI_Init(&argc, &argv);
+    {
+      const double dt = std::strtod(argv[3], nullptr);
+      const long nsteps = std::strtol(argv[4], nullptr, 10);
+      std::vector<double> ev;
+      if (std::string(argv[5]) != "NULL") ev = read_values(argv[5]);
+      const long nmore = std::strtol(argv[9], nullptr, 10);
+      {
+        EPH_FDM g(argv[2]);
+        g.set_comm(MPI_COMM_WORLD, 0, 1);
+        g.set_dt(dt);
+        for (long s = 1; s <= nsteps; ++s) {
+          for (size_t k = 0; k + 3 < ev.size(); k += 4) g.insert_energy(ev[k], ev[k + 1], ev[k + 2], ev[k + 3]);
+          g.solve();
+        }
+        g.save_state(argv[6]);
+        g.save_temperature(argv[7], std::atoi(argv[8]));
+      }
+      EPH_FDM r(argv[6]);
+      r.set_comm(MPI_COMM_WORLD, 0, 1);
+      r.set_dt(dt);
+      std::ifstream f(argv[6]);
+      std::string line;
+      for (int k = 0; k < 3; ++k) std::getline(f, line);
+      size_t nx, ny, nz, st; double x0, x1, y0, y1, z0, z1;
+      f >> nx >> ny >> nz >> st >> x0 >> x1 >> y0 >> y1 >> z0 >> z1;
+      const double dx = (x1 - x0) / nx, dy = (y1 - y0) / ny, dz = (z1 - z0) / nz;
+      for (long s = 1; s <= nmore; ++s) {
+        for (size_t k = 0; k + 3 < ev.size(); k += 4) r.insert_energy(ev[k], ev[k + 1], ev[k + 2], ev[k + 3]);
+        r.solve();
+      }
+      std::printf("STEP %ld\n", nmore);
+      for (size_t k = 0; k < nz; ++k)
+        for (size_t j = 0; j < ny; ++j)
+          for (size_t i = 0; i < nx; ++i)
+            std::printf("%a\n", r.get_T(x0 + (i + 0.5) * dx, y0 + (j + 0.5) * dy, z0 + (k + 0.5) * dz));
     }
     MPI_Finalize();
     return 0;

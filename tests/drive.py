@@ -66,6 +66,8 @@ class Pair:
         args = inputs.fix_args(beta, elements, seed=seed, flags=flags, grid=(grid, grid, grid),
                                T_infile=T_in, C_e=C_e, kappa=kappa, style=style, model=model,
                                tau0=tau0)
+        self.args = args
+        self.T_infile = T_in
         inf = self.md.info()
         lo, hi = inf["boxlo"], inf["boxhi"]
         masses = [inputs.MASS_CU] * species
@@ -128,3 +130,37 @@ def bitwise_fraction(a, b):
     a = np.asarray(a, float).ravel()
     b = np.asarray(b, float).ravel()
     return float(np.mean(a.view(np.int64) == b.view(np.int64))) if a.size else 1.0
+
+
+def oracle_run(oracle, ea, kw, args, species, dt, nsteps, rebuilt=None, first_step=1,
+               every=None, check=None):
+    """The oracle FixEPH on the host harness (the stand-in for LAMMPS) in
+    Verlet order for nsteps; the neighbour lists are rebuilt where `rebuilt`
+    says (one flag per step), or when an atom moved more than half the skin.
+    `check(step, snapshot, fix)` runs after every `every` steps.  Returns the
+    final snapshot and the fix."""
+    kw = dict(kw)
+    cells = kw.pop("cells")
+    md = ea.MD(cells, **kw)
+    inf = md.info()
+    ora = oracle.Fix(args, species, inf["boxlo"], inf["boxhi"], dt, inputs.BOLTZ, inputs.FTM2V,
+                     masses=[inputs.MASS_CU] * species)
+    s = md.snapshot()
+    md.lj(s)
+    for k in range(nsteps):
+        n = first_step + k
+        ora.initial_integrate(s)
+        if (rebuilt[k] if rebuilt is not None else md.needs_rebuild(s)):
+            md.rebuild(s)
+            v = s.v[:s.nlocal].copy()
+            s = md.snapshot()
+            s.v[:s.nlocal] = v
+        md.update_ghosts(s)
+        md.lj(s)
+        ora.post_force(s, step=n)
+        ora.final_integrate(s)
+        ora.end_of_step(s, n)
+        if check is not None and every and (k + 1) % every == 0:
+            check(n, s, ora)
+    return s, ora
+

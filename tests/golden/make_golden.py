@@ -200,6 +200,101 @@ def fdm_fixtures(tmp):
         print("wrote", name, snaps.shape)
 
 
+def fdm_save_fixtures(tmp):
+    """save_state (T.restart) and save_temperature (heat map) files of the
+    compiled EPH_FDM after a few solves, and its resume from the state file:
+    fdm_save.npz holds each case's input files, the two written files byte for
+    byte, and the T of the resumed grid.  The parameter file is named
+    relative to the working directory, as fix eph users give it."""
+    import subprocess
+    rng = np.random.default_rng(23)
+    d = {}
+    cases = {}
+    nx, ny, nz = 6, 5, 4
+    n = nx * ny * nz
+    cases["periodic"] = dict(nx=nx, ny=ny, nz=nz, box=(0.0, 6.0, -1.0, 4.0, 2.0, 10.0),
+                             T=rng.uniform(100, 900, n), S=np.zeros(n), rho=np.ones(n),
+                             C=np.full(n, 1.41e-4), kappa=np.full(n, 0.1), flag=np.ones(n, int),
+                             tdyn=np.zeros(n, int), steps=1, dt=1e-3, nsteps=5, nmore=3,
+                             energies=np.column_stack([rng.uniform(0, 6, 16), rng.uniform(-1, 4, 16),
+                                                       rng.uniform(2, 10, 16), rng.normal(0, 1e-3, 16)]))
+    nx, ny, nz = 7, 6, 5
+    n = nx * ny * nz
+    flag = np.ones(n, int)
+    flag[rng.choice(n, 20, replace=False)] = 2
+    flag[rng.choice(n, 10, replace=False)] = 0
+    cases["walls"] = dict(nx=nx, ny=ny, nz=nz, box=(0.0, 7.0, 0.0, 6.0, 0.0, 5.0),
+                          T=rng.uniform(200, 2000, n), S=rng.normal(0, 1e-2, n),
+                          rho=rng.uniform(0.5, 1.5, n), C=rng.uniform(1e-4, 3e-4, n),
+                          kappa=rng.uniform(0.05, 0.2, n), flag=flag, tdyn=np.zeros(n, int),
+                          steps=2, dt=5e-4, nsteps=4, nmore=2, energies=None)
+    nx, ny, nz = 5, 4, 3
+    n = nx * ny * nz
+    Tg = np.arange(400) * 10.0
+    cases["tdyn"] = dict(nx=nx, ny=ny, nz=nz, box=(0.0, 5.0, 0.0, 4.0, 0.0, 3.0),
+                         T=rng.uniform(300, 1500, n), S=rng.normal(0, 1e-3, n), rho=np.ones(n),
+                         C=np.full(n, 2e-5), kappa=np.full(n, 0.06), flag=np.ones(n, int),
+                         tdyn=(rng.uniform(size=n) < 0.6).astype(int), steps=1, dt=1e-3, nsteps=3,
+                         nmore=2, energies=None,
+                         params=(10.0, 1e-5 + 2e-8 * Tg + 1e-6 * np.sin(Tg / 300.0) ** 2,
+                                 0.05 + 1e-5 * Tg))
+    for name, c in cases.items():
+        work = os.path.join(tmp, "save_" + name)
+        os.makedirs(work)
+        params = "NULL"
+        if "params" in c:
+            params = name + ".params"
+            inputs.write_params(os.path.join(work, params), *c["params"])
+            with open(os.path.join(work, params), "rb") as f:
+                d[f"{name}_params"] = np.frombuffer(f.read(), np.uint8)
+        inputs.write_grid(os.path.join(work, name + ".in"), c["nx"], c["ny"], c["nz"], c["box"],
+                          c["T"], c["S"], c["rho"], c["C"], c["kappa"], c["flag"], c["tdyn"],
+                          steps=c["steps"], params=params)
+        efile = "NULL"
+        if c["energies"] is not None:
+            efile = "e.txt"
+            write_values(os.path.join(work, efile), np.asarray(c["energies"]).ravel())
+            d[f"{name}_energies"] = np.asarray(c["energies"])
+        out = subprocess.run([eo.REF_DRIVER, "fdm_save", name + ".in", repr(c["dt"]), str(c["nsteps"]),
+                              efile, "T.restart", "Te_out", "7", str(c["nmore"])],
+                             cwd=work, check=True, capture_output=True, text=True).stdout
+        resumed = [float.fromhex(l) for l in out.split("STEP ")[1].strip().split("\n")[1:]]
+        for key, fn in (("grid", name + ".in"), ("state", "T.restart"), ("heat", "Te_out_000007")):
+            with open(os.path.join(work, fn), "rb") as f:
+                d[f"{name}_{key}"] = np.frombuffer(f.read(), np.uint8)
+        d[f"{name}_dt"] = c["dt"]
+        d[f"{name}_nsteps"] = c["nsteps"]
+        d[f"{name}_nmore"] = c["nmore"]
+        d[f"{name}_resume_T"] = np.array(resumed)
+    d["cases"] = np.array(list(cases))
+    np.savez_compressed(os.path.join(HERE, "fdm_save.npz"), **d)
+    print("wrote fdm_save.npz")
+
+
+def legacy_grid_fixture(tmp):
+    """Examples/FDM/T.in is in the older grid layout (no parameter-file line,
+    nine columns per cell).  The compiled reference reads only the current
+    layout, so it runs the same grid converted to it (NULL parameter file,
+    T_dyn 0): fdm_legacy.npz holds the original file and T after each of four
+    solves with a few energies inserted before every solve."""
+    src = os.path.join(REF, "Examples/FDM/T.in")
+    with open(src) as f:
+        lines = f.read().split("\n")
+    conv = lines[:7] + ["NULL"] + [l + " 0" for l in lines[7:] if l.strip()]
+    cp = os.path.join(tmp, "legacy_converted.in")
+    with open(cp, "w") as f:
+        f.write("\n".join(conv) + "\n")
+    rng = np.random.default_rng(31)
+    e = np.column_stack([rng.uniform(0, 17.6, 6), rng.uniform(0, 17.6, 6), rng.uniform(0, 17.6, 6),
+                         rng.uniform(0.5, 2.0, 6)])
+    snaps = ref_fdm(cp, 1.0, 4, 1, e, tmp)
+    with open(src, "rb") as f:
+        raw = np.frombuffer(f.read(), np.uint8)
+    np.savez_compressed(os.path.join(HERE, "fdm_legacy.npz"), legacy_file=raw, energies=e, dt=1.0,
+                        T_steps=snaps)
+    print("wrote fdm_legacy.npz", snaps.shape)
+
+
 def outref_fixture():
     d = {}
     for t in (1, 2, 3, 4):
@@ -301,7 +396,9 @@ def main():
     tmp = tempfile.mkdtemp()
     if "--only" in sys.argv:  # e.g. --only kappa: write that family alone
         try:
-            {"kappa": kappa_fixtures}[sys.argv[sys.argv.index("--only") + 1]](tmp)
+            {"kappa": kappa_fixtures, "fdm_save": fdm_save_fixtures,
+             "fdm_legacy": legacy_grid_fixture}[
+                sys.argv[sys.argv.index("--only") + 1]](tmp)
         finally:
             shutil.rmtree(tmp)
         return
@@ -334,6 +431,8 @@ def main():
         linear_fixture(tmp)
         kappa_fixtures(tmp)
         fdm_fixtures(tmp)
+        fdm_save_fixtures(tmp)
+        legacy_grid_fixture(tmp)
         outref_fixture()
         kat = {"source": "Random123 kat_vectors, philox4x32_10 (Salmon et al., SC'11)",
                "vectors": [

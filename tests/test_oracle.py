@@ -157,6 +157,76 @@ def test_fdm_golden_bitwise(oracle, tmp_path, name):
         assert np.array_equal(g.get("T"), d["T_steps"][s]), f"step {s + 1}"
 
 
+def _save_case_files(d, name, where):
+    """the case's grid (and parameter) file of fdm_save.npz, written into `where`"""
+    with open(os.path.join(where, name + ".in"), "wb") as f:
+        f.write(d[f"{name}_grid"].tobytes())
+    if f"{name}_params" in d:
+        with open(os.path.join(where, name + ".params"), "wb") as f:
+            f.write(d[f"{name}_params"].tobytes())
+    return d[f"{name}_energies"] if f"{name}_energies" in d else None
+
+
+@pytest.mark.parametrize("name", ["periodic", "walls", "tdyn"])
+def test_fdm_writers_and_resume_match_reference(oracle, tmp_path, monkeypatch, name):
+    """EPH_FDM::save_state (T.restart, eph_fdm.h:226-265) and save_temperature
+    (the heat map, :198-224) written after a few solves are byte-identical to
+    the compiled reference's files, T-dependent C_e and kappa included; a grid
+    read back from the state file (eph_fdm.h:48-119) then continues exactly as
+    the reference's does."""
+    d = np.load(os.path.join(G, "fdm_save.npz"))
+    monkeypatch.chdir(tmp_path)  # the parameter file is named relative to it
+    E = _save_case_files(d, name, str(tmp_path))
+    dt = float(d[f"{name}_dt"])
+
+    def run(g, n):
+        g.set_dt(dt)
+        for _ in range(n):
+            if E is not None:
+                for x, y, z, e in E:
+                    g.insert_energy(x, y, z, e)
+            g.solve()
+
+    g = oracle.FDM.load(name + ".in")
+    run(g, int(d[f"{name}_nsteps"]))
+    g.save_state("T.restart")
+    g.save_temperature("Te_out", 7)
+    for fn, key in (("T.restart", "state"), ("Te_out_000007", "heat")):
+        with open(fn, "rb") as f:
+            got = f.read()
+        assert got == d[f"{name}_{key}"].tobytes(), f"{fn} differs from the reference's"
+    r = oracle.FDM.load("T.restart")
+    run(r, int(d[f"{name}_nmore"]))
+    assert np.array_equal(r.get("T"), d[f"{name}_resume_T"])
+
+
+def test_fdm_legacy_grid_layout(oracle, tmp_path):
+    """Examples/FDM/T.in (no parameter-file line, nine columns per cell) is
+    read as the current-layout grid with a NULL parameter file and T_dyn 0:
+    four solves match the compiled reference run on that converted grid."""
+    d = np.load(os.path.join(G, "fdm_legacy.npz"))
+    gp = str(tmp_path / "T.in")
+    with open(gp, "wb") as f:
+        f.write(d["legacy_file"].tobytes())
+    g = oracle.FDM.load(gp)
+    assert g.ntotal == 125
+    g.set_dt(float(d["dt"]))
+    for s in range(4):
+        for x, y, z, e in d["energies"]:
+            g.insert_energy(x, y, z, e)
+        g.solve()
+        assert np.array_equal(g.get("T"), d["T_steps"][s]), f"step {s + 1}"
+    # a line after the box that is neither a file name nor a legacy cell line
+    # is refused
+    bad = d["legacy_file"].tobytes().decode().split("\n")
+    bad[7] = "0 0 0 300.0 0.0"
+    bp = str(tmp_path / "bad.in")
+    with open(bp, "w") as f:
+        f.write("\n".join(bad))
+    with pytest.raises(Exception):
+        oracle.FDM.load(bp)
+
+
 def test_fdm_unstable_window_substeps(oracle, tmp_path):
     # eph_fdm.h:308-313 floors dt/inner: r = 0.765 gives ONE sub-step (lambda 0.255)
     d = np.load(os.path.join(G, "fdm_unstable.npz"))
@@ -690,6 +760,11 @@ def test_host_code_under_address_and_ub_sanitizers(tmp_path):
     T = 300.0 + 50.0 * np.arange(27)
     write_grid_fast(pg, 3, (0.0, 10.4, 0.0, 10.4, 0.0, 10.4), T, C_e=3e-6, kappa=0.1, rho=1.0)
     env = dict(os.environ, ASAN_OPTIONS="detect_leaks=0", OMP_NUM_THREADS="2")
-    r = subprocess.run([exe, pb, pk, pg], capture_output=True, text=True, timeout=300, env=env)
-    assert r.returncode == 0 and "san_driver: ok" in r.stdout, r.stderr[-3000:]
-    assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-3000:]
+    # the current grid layout, then the older one (Examples/FDM/T.in)
+    pl = str(tmp_path / "legacy.T")
+    with open(pl, "wb") as f:
+        f.write(np.load(os.path.join(G, "fdm_legacy.npz"))["legacy_file"].tobytes())
+    for grid in (pg, pl):
+        r = subprocess.run([exe, pb, pk, grid], capture_output=True, text=True, timeout=300, env=env)
+        assert r.returncode == 0 and "san_driver: ok" in r.stdout, r.stderr[-3000:]
+        assert "runtime error" not in r.stderr and "AddressSanitizer" not in r.stderr, r.stderr[-3000:]

@@ -179,6 +179,7 @@ int eph_adjoint::restore(const State& st) {
   HCPY(ctx->nbr.p, st.nbr.p, sf * sizeof(int));
   HCPY(ctx->nnbr.p, st.nnbr.p, ctx->stride * sizeof(int));
   ctx->drop_candidates();
+  ctx->pv_fresh = false;
   HCPY(md->lj_nbr.p, st.lj.p, sl * sizeof(int));
   HCPY(md->lj_nn.p, st.ljn.p, ctx->stride * sizeof(int));
   EPH_HIP(this, launch_fill_double(ctx->grid->dT.p, G, 0.0, s));
@@ -427,6 +428,7 @@ int eph_adjoint_forward(eph_adjoint* a, const double* beta, const double* v0, do
     for (int i = 0; i < a->nl; ++i) v4[i] = make_double4(v0[3 * i], v0[3 * i + 1], v0[3 * i + 2], 0.0);
     EPH_HIP(a, hipMemcpyAsync(ctx->vel.p, v4.data(), a->nl * sizeof(double4), hipMemcpyHostToDevice, s));
     EPH_HIP(a, hipStreamSynchronize(s));
+    ctx->pv_fresh = false;
   }
   for (int n = 1; n <= a->nsteps; ++n) {
     if ((n - 1) % a->every == 0 && (rc = a->save(a->ckpt[(n - 1) / a->every]))) return rc;

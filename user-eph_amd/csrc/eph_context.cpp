@@ -201,6 +201,9 @@ AtomsView eph_context::atoms_view() {
   a.nlocal = nlocal; a.nghost = nghost; a.stride = stride;
   a.pos = pos.p; a.vel = vel.p; a.f = f.p; a.fstride = fstride;
   a.type = type.p; a.mask = mask.p; a.tag = tag.p; a.groupbit = groupbit;
+  // eph/coloured/exp/v1 sweeps filtered velocities and eph/atomic packs its
+  // own: only the other styles keep the records current as atoms move
+  a.pv = (colour == 3 || atomic()) ? nullptr : pv.p;
   return a;
 }
 
@@ -214,7 +217,7 @@ StepParams eph_context::step_params() {
 }
 
 void eph_context::drop_candidates() {
-  if (cstate.p) (void)launch_fill_int(cstate.p, 2, 0, stream);
+  if (cstate.p) (void)launch_fill_int(cstate.p, 8, 0, stream);
 }
 
 WorkView eph_context::work_view() {
@@ -226,14 +229,17 @@ WorkView eph_context::work_view() {
   w.vw = colour == 3 ? colA.p : vel.p;
   const double delta = cand_delta < 0.0 ? 0.1 * beta.r_cutoff : cand_delta;
   if (delta > 0.0 && cstate.p && cand.p && cand.n >= act.n && ncand.n >= nact.n &&
-      cref.n >= static_cast<size_t>(stride)) {
-    w.cand = cand.p; w.ncand = ncand.p; w.cref = cref.p; w.cstate = cstate.p;
+      cref.n >= static_cast<size_t>(nlocal) + nghost && moved.n >= static_cast<size_t>(kMovedCap)) {
+    w.cand = cand.p; w.ncand = ncand.p; w.cref = cref.p; w.cstate = cstate.p; w.moved = moved.p;
+    w.moved_cap = kMovedCap;
     const double rc = beta.r_cutoff + delta;
     w.rcand2 = rc * rc;
-    // the candidates are retaken once an atom has moved 0.45 delta (not
-    // delta/2): entries left out are then farther than rc + 0.1 delta, a
-    // margin far above the rounding of the distances
+    // moved = 0.45 delta (not delta/2): an entry left out that neither atom
+    // of the pair has moved is farther than rc + 0.1 delta, and a moved atom
+    // is looked for within rc + 0.1 delta of a wave's atoms -- margins far
+    // above the rounding of the distances
     w.half_delta2 = (0.45 * delta) * (0.45 * delta);
+    w.reach = beta.r_cutoff + 0.1 * delta;
   }
   return w;
 }
@@ -269,8 +275,9 @@ int eph_context::reserve_atoms(int nl, int ng) {
   EPH_HIP(this, nnbr.reserve(st));
   EPH_HIP(this, nact.reserve(st));
   EPH_HIP(this, ncand.reserve(st));
-  EPH_HIP(this, cref.reserve(st));
-  EPH_HIP(this, cstate.reserve(4));
+  EPH_HIP(this, cref.reserve(nall));
+  EPH_HIP(this, moved.reserve(kMovedCap));
+  EPH_HIP(this, cstate.reserve(8));
   EPH_HIP(this, stage.reserve(3 * nall));
   EPH_HIP(this, xi_in.reserve(3 * (size_t)st));
   if (st != stride || f.n < 3 * (size_t)st) {
@@ -285,6 +292,7 @@ int eph_context::reserve_atoms(int nl, int ng) {
   fstride = st;
   nlocal = nl;
   nghost = ng;
+  pv_fresh = false;
   // the filter states (and the eph/atomic energies) cover every atom slot a
   // later device rebuild may fill
   if (atomic()) return eph::atomic_reserve(this, nall);
@@ -459,9 +467,12 @@ int eph_context::dev_post_force() {
   const WorkView w = work_view();
   hipEvent_t e;
   if (colour == 3 && (flags & EPH_FRICTION)) EPH_HIP(this, launch_colour_v(a, p, w, stream));
-  timed_begin("pack_pv", &e);
-  EPH_HIP(this, launch_pack_pv(a, w, stream));
-  timed_end("pack_pv", e);
+  if (!pv_fresh || !a.pv) {
+    timed_begin("pack_pv", &e);
+    EPH_HIP(this, launch_pack_pv(a, w, stream));
+    timed_end("pack_pv", e);
+    pv_fresh = a.pv != nullptr;
+  }
   // models other than PRL need rho (and xi) but no w sum
   StepParams pd = p;
   if (model != 4) pd.flags &= ~EPH_FRICTION;
@@ -562,6 +573,14 @@ int eph_context::dev_end_of_step(long long ntimestep) {
   return EPH_OK;
 }
 
+int eph_grid::bucket_bytes_ok(size_t G) {
+  EPH_HIP(this, bcount.reserve(G));  // zero-filled when first made
+  EPH_HIP(this, bitems.reserve(G * kBucket));
+  EPH_HIP(this, bqueue.reserve(G));
+  EPH_HIP(this, bnq.reserve(1));
+  return EPH_OK;
+}
+
 // the deposit half of end_of_step (fix_eph.cpp:364-404): dT_e and the tally
 int eph_context::dev_deposit() {
   if (!have_atoms) return fail(EPH_ERR_STATE, "no atoms");
@@ -569,6 +588,19 @@ int eph_context::dev_deposit() {
   const int nb = nblocks_of(nlocal);
   const size_t G = grid->h.ntotal();
   hipEvent_t e;
+  // a few atoms per cell on average: per-cell buckets (three launches)
+  if ((fric || rnd) && static_cast<double>(nlocal) <= 6.0 * static_cast<double>(G)) {
+    EPH_HIP(this, grid->dep.reserve(nlocal, 0, nb));
+    if (int rc = grid->bucket_bytes_ok(G)) return fail(rc, grid->err);
+    DepositView d = grid->dep.view(true);
+    BucketView b;
+    b.count = grid->bcount.p; b.items = grid->bitems.p; b.queue = grid->bqueue.p; b.nqueue = grid->bnq.p;
+    timed_begin("deposit", &e);
+    EPH_HIP(this, launch_deposit_bucket(atoms_view(), grid->v, step_params(), work_view(), d, b, fric, rnd,
+                                        scal.p, scal.p + 1, stream));
+    timed_end("deposit", e);
+    return EPH_OK;
+  }
   if (fric || rnd) {
     EPH_HIP(this, grid->dep.reserve(nlocal, G, nb));
     DepositView d = grid->dep.view(true);
@@ -928,6 +960,7 @@ int eph_context::d2h(void* dst, const void* src, size_t bytes) {
 }
 
 static int upload_rows(eph_context* c, const double* rows, int n, double4* dst) {
+  c->pv_fresh = false;  // positions or velocities written behind the packed records
   if (n == 0) return EPH_OK;
   if (int rc = c->h2d(c->stage.p, rows, 3 * (size_t)n * sizeof(double))) return rc;
   EPH_HIP(c, launch_rows_into_d4(c->stage.p, n, dst, c->stream));
@@ -1118,6 +1151,14 @@ int eph_get_state(eph_context* ctx, int which, double* out) {
       const int n = which == 10 ? ctx->nlocal : nall;
       if (n) EPH_HIP(ctx, hipMemcpyAsync(out, src, n * sizeof(double), hipMemcpyDeviceToHost, s));
       break;
+    }
+    case 12: {  // candidate-list statistics of the last cooperative density sweep
+      int st[8] = {};
+      if (ctx->cstate.p) EPH_HIP(ctx, hipMemcpyAsync(st, ctx->cstate.p, sizeof st, hipMemcpyDeviceToHost, s));
+      EPH_HIP(ctx, hipStreamSynchronize(s));
+      const int nwaves = (ctx->nlocal + 63) / 64;
+      out[0] = st[3]; out[1] = nwaves; out[2] = st[4]; out[3] = st[5];
+      return EPH_OK;
     }
     default: return ctx->fail(EPH_ERR_ARGS, "unknown state selector");
   }

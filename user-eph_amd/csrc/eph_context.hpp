@@ -51,6 +51,9 @@ struct eph_grid {
   eph::DevArray<double4> C_T, kappa_T;
   eph::DevArray<int> nonfinite, lookup_idx;
   eph::DepositBuffers dep;
+  // per-cell buckets of the deposit (count kept at zero between steps)
+  eph::DevArray<int> bcount, bitems, bqueue, bnq;
+  int bucket_bytes_ok(size_t G);  // reserves the buckets; 0 when they are ready
   bool any_tdyn = false;
   bool minmax_valid = false;
   double cmin = 0, rmin = 0, kmax = 0;
@@ -108,11 +111,15 @@ struct eph_context {
   // candidate list of the cooperative density sweep (WorkView::cand);
   // cand_delta < 0: 0.1 r_cutoff, 0: none
   eph::DevArray<int> cand, ncand, cstate;
-  eph::DevArray<double4> cref;
+  eph::DevArray<double4> cref, moved;
   double cand_delta = -1.0;
   // after any change of the lists, ghosts, types, masks or group
   void drop_candidates();
   int xi_mode_next = 0;  // 1 once eph_set_noise injected values
+  // the packed (position, velocity) records match pos and vel: the kernels
+  // that move atoms keep them so; anything else that writes pos or vel
+  // (uploads, restores) clears this and the next post_force repacks
+  bool pv_fresh = false;
   // kernel forms of the two sweeps (-1 chosen by system size; see eph_autotune)
   int form_density = -1, form_force = -1;
   long long step = 0;

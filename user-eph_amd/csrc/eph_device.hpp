@@ -55,6 +55,8 @@ struct GridView {
   const double* E_y = nullptr; const double* E_dy = nullptr; int n_E = 0; double E_dx = 1;
 };
 
+struct PosVel;
+
 struct AtomsView {
   int nlocal = 0, nghost = 0;
   int stride = 0;             // ELL column stride (>= nlocal, multiple of 64)
@@ -66,6 +68,9 @@ struct AtomsView {
   const int* mask = nullptr;
   const int* tag = nullptr;
   int groupbit = 1;
+  // when set, the kernels that move atoms (integrator, ghost update, remap)
+  // also keep the packed (position, velocity) records of the w sweep current
+  PosVel* pv = nullptr;
 };
 
 struct StepParams {
@@ -112,17 +117,30 @@ struct WorkView {
   double4* cB = nullptr;
   const double4* vw = nullptr;  // velocities the w sweep uses (vel, or zv)
   // Candidate list of the cooperative density sweep: the entries of the full
-  // list with r^2 < rcand2 = (rc + delta)^2 at the positions cref.  While no
-  // local atom has moved delta/2 from cref, every entry outside it has r > rc,
-  // so the rho loop may walk the candidates instead of the full list (same
-  // active entries, same order).  rcand2 = 0: not used.
+  // list with r^2 < rcand2 = (rc + delta)^2 at the positions cref, taken for
+  // every local atom in one sweep.  Until the next such sweep, an atom (local
+  // or ghost) whose distance from its cref reaches 0.45 delta is "moved"; the
+  // check before each sweep lists the current positions of the moved atoms.
+  // An entry j left out of atom i's candidates was at least rc + delta away,
+  // so it can be within rc now only if i or j moved, and then a moved atom
+  // lies within rc of i.  A wave none of whose atoms has a moved atom within
+  // `reach` (> rc) walks its candidates: the same active entries in the same
+  // order.  Any other wave scans the full list.  More than moved_cap moved
+  // atoms retakes the candidates of all atoms.  rcand2 = 0: not used.
   int* cand = nullptr;          // ELL, same stride and width as nbr
   int* ncand = nullptr;
-  double4* cref = nullptr;      // [nlocal]
-  int* cstate = nullptr;        // [0]: candidates valid, [1]: an atom moved >= delta/2
+  double4* cref = nullptr;      // [nlocal + nghost]
+  double4* moved = nullptr;     // [moved_cap] positions of the moved atoms
+  // [0] candidates valid, [1] waves that scanned the full list, [2] moved
+  // count; [3], [4] the same two of the last sweep, [5] 1 if it retook them
+  int* cstate = nullptr;
   double rcand2 = 0.0;
-  double half_delta2 = 0.0;     // (delta/2)^2
+  double half_delta2 = 0.0;     // (0.45 delta)^2
+  double reach = 0.0;           // rc + 0.1 delta
+  int moved_cap = 0;
 };
+// moved atoms listed per step before the candidates of all atoms are retaken
+constexpr int kMovedCap = 512;
 
 // Ordered deposit of per-atom values into grid cells (bit-identical to a
 // serial loop over atoms): cell of each item, counts, offsets and the
@@ -138,6 +156,17 @@ struct DepositView {
   double* partial = nullptr;  // [2 * nblocks] energy partial sums
   int* scan_tmp = nullptr;    // block sums for the scan
   int* maxcount = nullptr;    // [1]
+};
+
+// Per-cell buckets of the deposit for grids with a few atoms per cell
+// (launch_deposit_bucket): count and kBucket atom slots per cell, and the
+// queue of cells that received more atoms than that.
+constexpr int kBucket = 16;
+struct BucketView {
+  int* count = nullptr;   // [G], zero between steps
+  int* items = nullptr;   // [G][kBucket]
+  int* queue = nullptr;   // [G]
+  int* nqueue = nullptr;  // [1]
 };
 
 constexpr int kBlock = 256;
@@ -196,6 +225,12 @@ hipError_t launch_deposit_points(int n, const double* x, const double* y, const 
 // kMaxSortInCell items.  d.count must hold the per-cell counts.
 hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_items, int* qbuf,
                                 hipStream_t s);
+// the whole deposit and tally through per-cell buckets (d.v2 may be null for
+// no random part, d.v1 for no friction part); b.count must be zero on entry
+// and is left zero
+hipError_t launch_deposit_bucket(const AtomsView& a, const GridView& g, const StepParams& p,
+                                 const WorkView& w, const DepositView& d, const BucketView& b,
+                                 bool fric, bool rand, double* E_local, double* Ee, hipStream_t s);
 hipError_t launch_energy_reduce(const DepositView& d, int nblocks, double* E_local, double* Ee,
                                 hipStream_t s);
 // grid

@@ -289,7 +289,30 @@ bool load_grid(const std::string& path, GridHost& g, std::string& err) {
     return false;
   }
   g.set_box(bx[0], bx[1], bx[2], bx[3], bx[4], bx[5]);
-  in >> g.param_file;
+  // The line after the box names the parameter file (or NULL).  Older grid
+  // files (Examples/FDM/T.in) have no such line and nine columns per cell,
+  // "i j k T S rho C kappa flag", with no T-dependence flag: that layout is
+  // recognised by its first cell line and read with the flag at 0.
+  std::string rest, next;
+  std::getline(in, rest);  // the tokens may also follow the box on its line
+  next = rest;
+  while (in && next.find_first_not_of(" \t\r") == std::string::npos) std::getline(in, next);
+  std::vector<std::string> tok;
+  {
+    std::istringstream ls(next);
+    for (std::string t; ls >> t;) tok.push_back(t);
+  }
+  bool legacy = false;
+  if (tok.size() == 9) {
+    legacy = true;
+    g.param_file = "NULL";
+  } else if (tok.size() == 1) {
+    g.param_file = tok[0];
+  } else {
+    err = "unrecognised line after the box in grid file " + path;
+    return false;
+  }
+  std::istringstream first(legacy ? next : std::string());
   g.has_params = false;
   if (g.param_file != "NULL") {
     // the reference opens the name relative to the working directory; a name
@@ -331,17 +354,19 @@ bool load_grid(const std::string& path, GridHost& g, std::string& err) {
   }
   const size_t nt = g.ntotal();
   for (size_t c2 = 0; c2 < nt; ++c2) {
+    std::istream& src = (legacy && c2 == 0) ? static_cast<std::istream&>(first) : in;
     long lx, ly, lz;
-    in >> lx >> ly >> lz;
-    if (!in || lx < 0 || ly < 0 || lz < 0 || (size_t)lx >= g.nx || (size_t)ly >= g.ny ||
+    src >> lx >> ly >> lz;
+    if (!src || lx < 0 || ly < 0 || lz < 0 || (size_t)lx >= g.nx || (size_t)ly >= g.ny ||
         (size_t)lz >= g.nz) {
       err = "bad cell line in grid file " + path;
       return false;
     }
     size_t idx = lx + ly * g.nx + lz * g.nx * g.ny;
-    int fl, td;
-    in >> g.T[idx] >> g.S[idx] >> g.rho[idx] >> g.C[idx] >> g.kappa[idx] >> fl >> td;
-    if (!in) {
+    int fl, td = 0;
+    src >> g.T[idx] >> g.S[idx] >> g.rho[idx] >> g.C[idx] >> g.kappa[idx] >> fl;
+    if (!legacy) src >> td;
+    if (!src) {
       err = "truncated grid file " + path;
       return false;
     }

@@ -21,8 +21,16 @@ __global__ __launch_bounds__(kBlock) void k_ghost_xv(AtomsView a, const int* __r
   const int o = owner[g];
   const double4 p = a.pos[o];
   const double4 s = shift[g];
-  a.pos[a.nlocal + g] = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, p.w);
-  a.vel[a.nlocal + g] = a.vel[o];
+  const double4 x = make_double4(p.x + s.x, p.y + s.y, p.z + s.z, p.w);
+  const double4 v = a.vel[o];
+  a.pos[a.nlocal + g] = x;
+  a.vel[a.nlocal + g] = v;
+  if (a.pv) {
+    PosVel q;
+    q.p = x;
+    q.v = v;
+    a.pv[a.nlocal + g] = q;
+  }
 }
 
 // CSR -> transposed ELL; an index outside [0, nall) raises *bad and is stored
@@ -262,6 +270,34 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
   density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
 }
 
+// Is an atom listed as moved within w.reach of any of this wave's atoms?  The
+// moved positions are tested against the wave's bounding box grown by reach,
+// a superset of the spheres around its atoms; lanes share the list.
+__device__ __forceinline__ bool wave_near_moved(const bool valid, const double4& pi,
+                                                const WorkView& w, const int nm) {
+  const double inf = __builtin_huge_val();
+  double lx = valid ? pi.x : inf, ly = valid ? pi.y : inf, lz = valid ? pi.z : inf;
+  double hx = valid ? pi.x : -inf, hy = valid ? pi.y : -inf, hz = valid ? pi.z : -inf;
+  for (int o = 32; o > 0; o >>= 1) {
+    lx = fmin(lx, __shfl_xor(lx, o)); ly = fmin(ly, __shfl_xor(ly, o)); lz = fmin(lz, __shfl_xor(lz, o));
+    hx = fmax(hx, __shfl_xor(hx, o)); hy = fmax(hy, __shfl_xor(hy, o)); hz = fmax(hz, __shfl_xor(hz, o));
+  }
+  const double r = w.reach;
+  lx -= r; ly -= r; lz -= r;
+  hx += r; hy += r; hz += r;
+  const int lane = threadIdx.x & 63;
+  bool hit = false;
+  const int n = nm < w.moved_cap ? nm : w.moved_cap;
+  for (int b = 0; b < n; b += 64) {
+    const int m = b + lane;
+    if (m < n) {
+      const double4 q = w.moved[m];
+      hit = hit || (q.x >= lx && q.x <= hx && q.y >= ly && q.y <= hy && q.z >= lz && q.z <= hz);
+    }
+  }
+  return __any(hit) != 0;
+}
+
 // Cooperative form of density_w_atom (same sums in the same order): the
 // positions of the rho loop and the (position, velocity) records of the w loop
 // come through LDS, see coop_fetch_pos / coop_fetch_pv.  The records of entry
@@ -277,12 +313,17 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   const int lane = threadIdx.x & 63;
   const double4 pi = a.pos[i];
   const bool grp = valid && (a.mask[i] & a.groupbit) != 0;
-  // the rho loop walks the candidate list when it is valid and no atom has
-  // moved delta/2 since it was taken; otherwise the full list, taking the
-  // candidates on the way (when a candidate list is kept at all)
+  // the rho loop walks the candidate list when it is valid and no moved atom
+  // is near this wave; it scans the full list otherwise, and takes the
+  // candidates on the way when they are being retaken for all atoms
   const bool keep = w.rcand2 > 0.0;
-  const bool use_cand = keep && w.cstate[0] != 0 && w.cstate[1] == 0;
-  const bool take = keep && !use_cand;
+  bool use_cand = false, take = false;
+  if (keep) {
+    const int nm = w.cstate[2];
+    take = w.cstate[0] == 0 || nm > w.moved_cap;
+    use_cand = !take && !(nm > 0 && wave_near_moved(valid, pi, w, nm));
+    if (!use_cand && lane == 0) atomicAdd(w.cstate + 1, 1);
+  }
   const int* __restrict__ list = use_cand ? w.cand : w.nbr;
   const int nn = grp ? (use_cand ? w.ncand[i] : w.nnbr[i]) : 0;
   const int nnmax = wave_max(nn);
@@ -343,10 +384,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
     }
   }
   if (valid) w.nact[i] = na;
-  if (take && valid) {
-    w.ncand[i] = nc;
-    w.cref[i] = pi;
-  }
+  if (take && valid) w.ncand[i] = nc;
   const double alpha = density_alpha<MULTI>(grp, rho, pi, t, w);
   double wx = 0.0, wy = 0.0, wz = 0.0;
   if (FRIC) {
@@ -386,26 +424,37 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   if (valid) density_finish<XI>(i, grp, pi, rho, alpha, wx, wy, wz, a, p, w);
 }
 
-// Before the sweep: did any local atom move delta/2 or more since the
-// candidates were taken (a non-finite distance counts as moved)?
-__global__ __launch_bounds__(kBlock) void k_cand_check(int nlocal, const double4* __restrict__ pos,
+// Before the sweep, over all nall = nlocal + nghost atoms: on a step that
+// retakes the candidates, the reference positions are the current ones;
+// otherwise every atom 0.45 delta or farther from its reference position (a
+// non-finite distance counts) puts its current position on the moved list.
+__global__ __launch_bounds__(kBlock) void k_cand_check(int nall, const double4* __restrict__ pos,
                                                        WorkView w) {
-  if (w.cstate[0] == 0) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
-  bool moved = false;
-  if (i < nlocal) {
-    const double4 x = pos[i], r = w.cref[i];
-    const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
-    moved = !(dx * dx + dy * dy + dz * dz < w.half_delta2);
+  if (w.cstate[0] == 0) {
+    if (i < nall) w.cref[i] = pos[i];
+    return;
   }
-  if (__any(moved) && (threadIdx.x & 63) == 0) atomicOr(w.cstate + 1, 1);
+  if (i >= nall) return;
+  const double4 x = pos[i], r = w.cref[i];
+  const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+  if (!(dx * dx + dy * dy + dz * dz < w.half_delta2)) {
+    const int k = atomicAdd(w.cstate + 2, 1);
+    if (k < w.moved_cap) w.moved[k] = x;
+  }
 }
 
-// After the sweep: a sweep that walked the full list has taken the candidates
-__global__ void k_cand_flags(int* cstate) {
+// After the sweep: a sweep that retook the candidates leaves them valid; the
+// moved list starts empty at the next check
+__global__ void k_cand_flags(int* cstate, int moved_cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  if (cstate[0] == 0 || cstate[1] != 0) cstate[0] = 1;
+  const int retake = cstate[0] == 0 || cstate[2] > moved_cap;
+  cstate[3] = cstate[1];
+  cstate[4] = cstate[2];
+  cstate[5] = retake;
+  if (retake) cstate[0] = 1;
   cstate[1] = 0;
+  cstate[2] = 0;
 }
 
 // 256 atoms per block, tables through the caches, cooperative gathers
@@ -903,6 +952,14 @@ __global__ __launch_bounds__(kBlock) void k_integrate(AtomsView a, StepParams p)
     x.y += p.dtv * v.y;
     x.z += p.dtv * v.z;
     a.pos[i] = x;
+    if (a.pv) {
+      PosVel q;
+      q.p = x;
+      q.v = v;
+      a.pv[i] = q;
+    }
+  } else if (a.pv) {
+    a.pv[i].v = v;
   }
 }
 
@@ -1190,6 +1247,152 @@ __global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, i
   __shared__ double sh[1024];
   energy_reduce(partial, nb, E_local, Ee, sh);
 }
+
+// ----------------------------------------------- bucketed deposit
+// The same deposit for grids with a few atoms per cell (fcc at one cell per
+// unit cell holds 4 on average, at most 14 while every atom stays within
+// half a cell of its site).  Each cell collects its atoms in a bucket of
+// kBucket slots as they are counted; one thread per cell then sorts its
+// bucket and adds the values in ascending atom order, friction then random,
+// the serial order of fix_eph.cpp:364-390.  A cell that receives more atoms
+// than its bucket holds is queued and summed by k_deposit_finish from an
+// ordered scan of every atom's cell.  Three launches in all.
+template <bool FRIC, bool RAND>
+__global__ __launch_bounds__(kBlock) void k_deposit_bucket(AtomsView a, GridView g, StepParams p,
+                                                           WorkView w, DepositView d,
+                                                           BucketView b) {
+  __shared__ double sh[kBlock];
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i == 0) *b.nqueue = 0;  // the queue of this step starts empty
+  double eF = 0.0, eR = 0.0;
+  if (i < a.nlocal) {
+    int c = -1;
+    double v1 = 0.0, v2 = 0.0;
+    if (a.mask[i] & a.groupbit) {
+      const double4 x = a.pos[i];
+      const double4 v = a.vel[i];
+      c = cell_of(g, x.x, x.y, x.z);
+      const int fs = a.fstride;
+      const double prescale = g.dV * p.dt;
+      if (FRIC) {
+        double dE = 0.0;
+        dE -= w.fE[i] * v.x * p.dt;
+        dE -= w.fE[fs + i] * v.y * p.dt;
+        dE -= w.fE[2 * fs + i] * v.z * p.dt;
+        eF = dE;
+        v1 = dE / prescale;
+      }
+      if (RAND) {
+        double dE = 0.0;
+        dE -= w.fR[i] * v.x * p.dt;
+        dE -= w.fR[fs + i] * v.y * p.dt;
+        dE -= w.fR[2 * fs + i] * v.z * p.dt;
+        eR = dE;
+        v2 = dE / prescale;
+      }
+      const int slot = atomicAdd(&b.count[c], 1);
+      if (slot < kBucket) b.items[static_cast<size_t>(c) * kBucket + slot] = i;
+    }
+    d.cell[i] = c;
+    d.v1[i] = v1;
+    d.v2[i] = v2;
+  }
+  const double sF = block_sum(eF, sh);
+  const double sR = block_sum(eR, sh);
+  if (threadIdx.x == 0) {
+    d.partial[2 * blockIdx.x] = sF;
+    d.partial[2 * blockIdx.x + 1] = sR;
+  }
+}
+
+// compare-exchange of a fixed-size register array (indices known after
+// unrolling, so the array stays in registers)
+static __device__ __forceinline__ void cswap(int* k, int a, int b) {
+  const int x = k[a], y = k[b];
+  k[a] = min(x, y);
+  k[b] = max(x, y);
+}
+
+// one thread per cell: bitonic sort of the bucket (empty slots as INT_MAX),
+// then the ordered sums into dT_e; the bucket count is left at zero
+__global__ __launch_bounds__(kBlock) void k_deposit_bucket_cells(GridView g, DepositView d,
+                                                                 BucketView b, int ncell) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= ncell) return;
+  const int n = b.count[c];
+  if (n == 0) return;
+  b.count[c] = 0;
+  if (n > kBucket) {
+    b.queue[atomicAdd(b.nqueue, 1)] = c;
+    return;
+  }
+  int k[kBucket];
+  const int* src = b.items + static_cast<size_t>(c) * kBucket;
+#pragma unroll
+  for (int e = 0; e < kBucket; ++e) k[e] = e < n ? src[e] : 0x7fffffff;
+#pragma unroll
+  for (int size = 2; size <= kBucket; size <<= 1)
+#pragma unroll
+    for (int stride = size >> 1; stride > 0; stride >>= 1)
+#pragma unroll
+      for (int e = 0; e < kBucket; ++e) {
+        const int l = e ^ stride;
+        if (l > e) {
+          if ((e & size) == 0) cswap(k, e, l);
+          else cswap(k, l, e);
+        }
+      }
+  double acc = g.dT[c];
+  if (d.v1) {
+#pragma unroll
+    for (int e = 0; e < kBucket; ++e)
+      if (e < n) acc += d.v1[k[e]];
+  }
+  if (d.v2) {
+#pragma unroll
+    for (int e = 0; e < kBucket; ++e)
+      if (e < n) acc += d.v2[k[e]];
+  }
+  g.dT[c] = acc;
+}
+
+// block 0: E_local and Ee from the block partial sums (as k_energy_reduce);
+// blocks 1..: the queued cells, each by an ordered scan over all atoms:
+// every wave ballots the atoms of the cell among its 64, and thread 0 adds
+// their values in ascending order
+__global__ __launch_bounds__(1024) void k_deposit_finish(GridView g, DepositView d, BucketView b,
+                                                         int n_items, int nblocks, double* E_local,
+                                                         double* Ee) {
+  __shared__ double sh[1024];
+  __shared__ unsigned long long hits[16];
+  if (blockIdx.x == 0) {
+    energy_reduce(d.partial, nblocks, E_local, Ee, sh);
+    return;
+  }
+  const int nq = *b.nqueue;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int q = blockIdx.x - 1; q < nq; q += gridDim.x - 1) {
+    const int c = b.queue[q];
+    double acc = g.dT[c];  // meaningful in thread 0 only
+    for (int pass = 0; pass < 2; ++pass) {
+      const double* v = pass == 0 ? d.v1 : d.v2;
+      if (!v) continue;
+      for (int base = 0; base < n_items; base += 1024) {
+        const int i = base + threadIdx.x;
+        const unsigned long long m = __ballot(i < n_items && d.cell[i] == c);
+        if (lane == 0) hits[wv] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+          for (int w2 = 0; w2 < 16; ++w2)
+            for (unsigned long long bits = hits[w2]; bits; bits &= bits - 1)
+              acc += v[base + 64 * w2 + __builtin_ctzll(bits)];
+        __syncthreads();
+      }
+    }
+    if (threadIdx.x == 0) g.dT[c] = acc;
+  }
+}
+
 
 // ------------------------------------------------------------- grid
 // C_e(T), kappa(T) of temperature-dependent cells (eph_fdm.h:280-287)
@@ -1769,7 +1972,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   const bool keep = w.rcand2 > 0.0 && w.cstate;
   WorkView wc = w;
   if (!keep) wc.rcand2 = 0.0;
-  if (keep) k_cand_check<<<g, kBlock, 0, s>>>(a.nlocal, a.pos, wc);
+  if (keep) k_cand_check<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal + a.nghost, a.pos, wc);
 #define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, wc)
 #define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
 #define EPH_DW_F(XI) if (fric) { EPH_DW_M(XI, true); } else { EPH_DW_M(XI, false); }
@@ -1781,7 +1984,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
 #undef EPH_DW_F
 #undef EPH_DW_M
 #undef EPH_DW
-  if (keep) k_cand_flags<<<1, 64, 0, s>>>(w.cstate);
+  if (keep) k_cand_flags<<<1, 64, 0, s>>>(w.cstate, w.moved_cap);
   return hipGetLastError();
 }
 
@@ -1925,6 +2128,26 @@ hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_i
   k_cell_sum_small<<<nblk(G), kBlock, 0, s>>>(g, d, G, queue, qn);
   k_cell_sum_medium<<<64, kMedBlock, 0, s>>>(g, d, queue, qn, huge, hn);
   k_cell_sum_huge<<<16, kBlock, 0, s>>>(g, d, n_items, huge, hn);
+  return hipGetLastError();
+}
+
+hipError_t launch_deposit_bucket(const AtomsView& a, const GridView& g, const StepParams& p,
+                                 const WorkView& w, const DepositView& d, const BucketView& b,
+                                 bool fric, bool rand, double* E_local, double* Ee, hipStream_t s) {
+  const int G = g.nx * g.ny * g.nz;
+  const int nb = nblk(a.nlocal);
+  if (a.nlocal > 0) {
+    if (fric && rand) k_deposit_bucket<true, true><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
+    else if (fric) k_deposit_bucket<true, false><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
+    else k_deposit_bucket<false, true><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
+  } else {
+    k_fill_int<<<1, 64, 0, s>>>(b.nqueue, 1, 0);
+  }
+  DepositView dc = d;
+  if (!fric) dc.v1 = nullptr;
+  if (!rand) dc.v2 = nullptr;
+  k_deposit_bucket_cells<<<nblk(G), kBlock, 0, s>>>(g, dc, b, G);
+  k_deposit_finish<<<1 + 16, 1024, 0, s>>>(g, dc, b, a.nlocal, a.nlocal > 0 ? nb : 0, E_local, Ee);
   return hipGetLastError();
 }
 

@@ -228,6 +228,7 @@ int eph_md::upload(eph_context* ctx) {
   NbBuild& w = nbw;
   w.nlocal = nlocal; w.ghost_cap = ghost_cap; w.nbins = nbins;
   w.at.pos = ctx->pos.p; w.at.vel = ctx->vel.p; w.at.type = ctx->type.p; w.at.mask = ctx->mask.p;
+  w.at.pv = ctx->atoms_view().pv;
   w.at.tag = ctx->tag.p; w.at.rep = rep.p; w.at.owner = ctx->ghost_owner.p; w.at.shift = ctx->ghost_shift.p;
   w.box.lo = box_d.p; w.box.hi = box_d.p + 3; w.box.L = box_d.p + 6; w.box.glo = box_d.p + 9;
   w.box.nb = nbi.p; w.box.bin_base = nbi.p + 3; w.box.bs = bs; w.box.cg = cmax;

@@ -41,8 +41,8 @@ __global__ __launch_bounds__(kBlock) void k_nb_check(int nlocal, const double4* 
 }
 
 // Domain::remap of the locals
-__global__ __launch_bounds__(kBlock) void k_nb_wrap(int nlocal, double4* pos, const int* rep,
-                                                    NbBoxes b, const int* go) {
+__global__ __launch_bounds__(kBlock) void k_nb_wrap(int nlocal, double4* pos, PosVel* pv,
+                                                    const int* rep, NbBoxes b, const int* go) {
   if (!*go) return;
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i >= nlocal) return;
@@ -55,8 +55,10 @@ __global__ __launch_bounds__(kBlock) void k_nb_wrap(int nlocal, double4* pos, co
     if (c[d] >= hi) c[d] -= L;
     if (c[d] < lo) c[d] = lo;
   }
+  if (c[0] == p.x && c[1] == p.y && c[2] == p.z) return;
   p.x = c[0]; p.y = c[1]; p.z = c[2];
   pos[i] = p;
+  if (pv) pv[i].p = p;
 }
 
 // images of source s in dimension d: count (0..2)
@@ -346,7 +348,7 @@ hipError_t launch_nb_force(int* go, hipStream_t s) {
 hipError_t launch_nb_build(const NbBuild& w, hipStream_t s) {
   const int cap = w.nlocal + w.ghost_cap;
   const int* go = w.go;
-  k_nb_wrap<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.at.rep, w.box, go);
+  k_nb_wrap<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.at.pv, w.at.rep, w.box, go);
   // ghost passes: nghost[0] = 0, nghost[d + 1] = after pass d
   k_nb_zero<<<1, 64, 0, s>>>(w.nghost, 4, go);
   for (int d = 0; d < 3; ++d) {

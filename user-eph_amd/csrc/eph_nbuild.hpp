@@ -18,6 +18,7 @@ struct NbBoxes {
 };
 
 struct NbAtoms {
+  PosVel* pv = nullptr;  // packed records kept current by the remap (may be null)
   double4* pos = nullptr;
   double4* vel = nullptr;
   int* type = nullptr;

@@ -371,6 +371,14 @@ class FixEPH:
         self._check(lib().eph_get_state(self.h, idx, _d(out)))
         return out
 
+    def candidate_stats(self):
+        """The density sweep's candidate list at its last run: waves that
+        scanned the full list, waves in all, atoms found moved, retaken (bool)"""
+        out = np.zeros(4)
+        self._check(lib().eph_get_state(self.h, 12, _d(out)))
+        return {"full_waves": int(out[0]), "waves": int(out[1]), "moved": int(out[2]),
+                "retaken": bool(out[3])}
+
     # device-resident hooks
     def dev_initial_integrate(self): self._check(lib().eph_dev_initial_integrate(self.h))
     def dev_post_force(self): self._check(lib().eph_dev_post_force(self.h))

@@ -180,3 +180,43 @@ def workload_beta(path, species):
 def ensure_dir(p):
     os.makedirs(p, exist_ok=True)
     return p
+
+
+def write_uniform_grid(path, n, L, steps, T=300.0):
+    """n^3 T_e grid over a cubic box [0, L), uniform T, the given initial
+    sub-step count, constant C_e = 3e-6 and kappa = 0.1"""
+    idx = np.indices((n, n, n))[::-1].reshape(3, -1).T
+    with open(path, "w") as f:
+        f.write(f"# T_e grid: uniform {T} K\n#\n#\n")
+        f.write(f"{n} {n} {n} {steps}\n")
+        for _ in range(3):
+            f.write(f"0.0 {L!r}\n")
+        f.write("NULL\n")
+        f.write("\n".join(f"{i} {j} {k} {T!r} 0.0 1.0 3e-06 0.1 1 0" for i, j, k in idx.tolist()))
+        f.write("\n")
+
+
+ENSEMBLE_A = 3.615
+ENSEMBLE_DT = 2.5e-4  # 1 fs integrates no 1 keV atom with this lj/cut: it overruns a neighbour
+
+
+def ensemble_replicas(tmp, nrep=64, size_seed=3):
+    """The batched-ensemble workload: nrep replicas of the fcc plain-method
+    physics, n^3 cells with n = round(exp U[ln 10, ln 25]) (seeded), each with
+    its own n^3 T_e grid (uniform 300 K, 19 sub-steps), fix seed 7000 + r,
+    lattice seed 900 + r and a 1 keV primary atom.  Returns the list of
+    (harness keywords, fix arguments) and the sizes."""
+    rng = np.random.default_rng(size_seed)
+    ns = np.rint(np.exp(rng.uniform(np.log(10.0), np.log(25.0), nrep))).astype(int)
+    beta = os.path.join(tmp, "ens.beta")
+    names = workload_beta(beta, 1)
+    reps = []
+    for r, n in enumerate(ns.tolist()):
+        tin = os.path.join(tmp, f"ens{r}.T")
+        write_uniform_grid(tin, n, n * ENSEMBLE_A, 19)
+        fargs = fix_args(beta, names, seed=7000 + r, flags="7", grid=(n, n, n), T_infile=tin,
+                         C_e=3e-6, kappa=0.1)
+        kw = dict(cells=n, a=ENSEMBLE_A, ntypes=1, seed=900 + r, T=300.0, pka_eV=1000.0)
+        reps.append((kw, fargs))
+    return reps, ns
+
