@@ -27,3 +27,20 @@ def oracle():
     import eph_oracle
     eph_oracle.lib()
     return eph_oracle
+
+
+# Figures the parity tests measure (errors against the oracle per workload),
+# printed after the session's summary so that they stay in a run's tail.
+_REPORT = []
+
+
+@pytest.fixture(scope="session")
+def parity_report():
+    return _REPORT
+
+
+def pytest_terminal_summary(terminalreporter):
+    if _REPORT:
+        terminalreporter.write_sep("-", "parity figures")
+        for line in _REPORT:
+            terminalreporter.write_line(line)

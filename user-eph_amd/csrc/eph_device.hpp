@@ -68,8 +68,9 @@ struct AtomsView {
   const int* mask = nullptr;
   const int* tag = nullptr;
   int groupbit = 1;
-  // when set, the kernels that move atoms (integrator, ghost update, remap)
-  // also keep the packed (position, velocity) records of the w sweep current
+  // when set, the kernels that move atoms before post_force (initial
+  // integrate, ghost update, remap) also keep the packed (position,
+  // velocity) records of the w sweep current
   PosVel* pv = nullptr;
 };
 

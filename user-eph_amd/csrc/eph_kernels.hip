@@ -958,9 +958,10 @@ __global__ __launch_bounds__(kBlock) void k_integrate(AtomsView a, StepParams p)
       q.v = v;
       a.pv[i] = q;
     }
-  } else if (a.pv) {
-    a.pv[i].v = v;
   }
+  // the final kick leaves the records at (x', v_half) of this step's
+  // post_force (what the trajectory adjoint reads back); the next initial
+  // kick rewrites them before any sweep reads them
 }
 
 __global__ void k_fill_int(int* p, int n, int v) {

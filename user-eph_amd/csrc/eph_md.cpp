@@ -557,11 +557,23 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     bool host_rebuild = false;
     if (md->device_rebuild) {
       // neighbour check and, when an atom moved more than skin/2, the device
-      // rebuild of ghosts and both lists
+      // rebuild of ghosts and both lists (and the density sweep's candidates,
+      // at the margin in effect now)
+      const WorkView cw = ctx->work_view();
+      md->nbw.lists.cand = cw.rcand2 > 0.0 ? cw.cand : nullptr;
+      md->nbw.lists.ncand = cw.ncand;
+      md->nbw.lists.cand2 = cw.rcand2;
+      md->nbw.cref = cw.cref;
       EPH_HIP(md, launch_nb_check(md->nlocal, ctx->pos.p, md->xref.p, lim, md->nbw.go, s));
       if (md->force_next) EPH_HIP(md, launch_nb_force(md->nbw.go, s));
       md->force_next = false;
-      EPH_HIP(md, launch_nb_build(md->nbw, s));
+      const hipError_t eb = launch_nb_build(md->nbw, s);
+      // the candidate buffers belong to this step's launch only (the
+      // context may reallocate them before the next)
+      md->nbw.lists.cand = nullptr;
+      md->nbw.lists.ncand = nullptr;
+      md->nbw.cref = nullptr;
+      EPH_HIP(md, eb);
       EPH_HIP(md, hipMemcpyAsync(md->pinned, md->flags.p, 8 * sizeof(int), hipMemcpyDeviceToHost, s));
       EPH_HIP(md, hipStreamSynchronize(s));
       if (md->pinned[0]) {

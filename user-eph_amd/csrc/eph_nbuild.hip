@@ -194,7 +194,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
   const int bx = bin_coord(pi.x, glo[0], b.bs, nb[0]);
   const int by = bin_coord(pi.y, glo[1], b.bs, nb[1]);
   const int bz = bin_coord(pi.z, glo[2], b.bs, nb[2]);
-  int nf = 0, nl = 0;
+  int nf = 0, nl = 0, nc = 0;
   for (int dz = -2; dz <= 2; ++dz) {
     const int z = bz + dz;
     if (z < 0 || z >= nb[2]) continue;
@@ -214,6 +214,10 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
           if (r2 < l.cf2) {
             if (nf < l.cap_f) l.nbr_f[nf * l.stride + i] = j;
             ++nf;
+            if (l.cand && r2 < l.cand2) {
+              if (nc < l.cap_f) l.cand[nc * l.stride + i] = j;
+              ++nc;
+            }
           }
           if (r2 < l.cl2) {
             if (nl < l.cap_l) l.nbr_l[nl * l.stride + i] = j;
@@ -226,15 +230,22 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
   if (nf > l.cap_f || nl > l.cap_l) atomicOr(overflow, 2);
   l.nn_f[i] = nf < l.cap_f ? nf : l.cap_f;
   l.nn_l[i] = nl < l.cap_l ? nl : l.cap_l;
+  if (l.cand) l.ncand[i] = nc < l.cap_f ? nc : l.cap_f;
 }
 
-__global__ __launch_bounds__(kBlock) void k_nb_finish(int nlocal, const double4* __restrict__ pos,
+__global__ __launch_bounds__(kBlock) void k_nb_finish(int nlocal, const int* ng,
+                                                      const double4* __restrict__ pos,
                                                       double4* ref, int* go, int* nreb,
-                                                      int* cand_valid) {
+                                                      int* cand_valid, double4* cref,
+                                                      bool cand_written) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (!*go) return;
   if (i < nlocal) ref[i] = pos[i];
-  if (i == 0 && cand_valid) *cand_valid = 0;  // new lists: candidates are retaken
+  // the candidates of the new lists, if written, are valid at the positions
+  // of this rebuild (locals and the ghosts just made); otherwise the next
+  // density sweep retakes them
+  if (cand_written && i < nlocal + *ng) cref[i] = pos[i];
+  if (i == 0 && cand_valid) *cand_valid = cand_written ? 1 : 0;
   // the last block clears the flag only after every block has read it: the
   // flag is cleared by the next step's k_nb_reset instead
   if (i == 0) atomicAdd(nreb, 1);
@@ -370,8 +381,9 @@ hipError_t launch_nb_build(const NbBuild& w, hipStream_t s) {
   k_nb_bin_sort<<<nbk(w.nbins), kBlock, 0, s>>>(w.nbins, w.bstart, w.items, go);
   k_nb_lists<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.at.rep, w.box, w.bstart,
                                               w.items, w.lists, w.overflow, go);
-  k_nb_finish<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.xref, w.go, w.nrebuild,
-                                                w.cand_valid);
+  const bool cw = w.lists.cand && w.cref && w.cand_valid;
+  k_nb_finish<<<nbk(cw ? cap : w.nlocal), kBlock, 0, s>>>(w.nlocal, ng, w.at.pos, w.xref, w.go,
+                                                          w.nrebuild, w.cand_valid, w.cref, cw);
   return hipGetLastError();
 }
 

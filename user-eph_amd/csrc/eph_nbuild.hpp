@@ -33,6 +33,9 @@ struct NbLists {
   int* nbr_f = nullptr; int* nn_f = nullptr; int cap_f = 0; double cf2 = 0;  // fix list
   int* nbr_l = nullptr; int* nn_l = nullptr; int cap_l = 0; double cl2 = 0;  // pair list
   int stride = 0;
+  // the density sweep's candidates (entries of the fix list with r^2 < cand2,
+  // in list order, capacity cap_f), written with the lists when cand is set
+  int* cand = nullptr; int* ncand = nullptr; double cand2 = 0;
 };
 
 struct NbBuild {
@@ -44,7 +47,11 @@ struct NbBuild {
   int* nghost = nullptr;    // [4] ghosts after each pass; [3] is the result
   int* overflow = nullptr;  // [1] bit 0 ghosts, bit 1 lists
   int* nrebuild = nullptr;  // [1] count of device rebuilds
-  int* cand_valid = nullptr;  // [1] cleared by a rebuild (the density sweep's candidates)
+  // [1] the density sweep's candidate state: a rebuild that writes the
+  // candidates (lists.cand) marks them valid with cref = the positions of the
+  // rebuild; one that does not clears it
+  int* cand_valid = nullptr;
+  double4* cref = nullptr;  // [nlocal + ghost_cap]
   int* cnt = nullptr;       // [nlocal + ghost_cap]
   int* start = nullptr;     // [nlocal + ghost_cap + 1]
   int* bin_of = nullptr;    // [nlocal + ghost_cap]
