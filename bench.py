@@ -59,6 +59,25 @@ def write_inputs(tmp, cells, a, grid, species):
     return beta, names, tin
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "fix_eph_hbm_counters_fcc80_cascade.txt")
+
+
+def counter_traffic(natoms):
+    """HBM bytes per launch of each kernel of the 2,048,000-atom cascade, from
+    the committed FETCH_SIZE / WRITE_SIZE counter summary (two rocprofv3 --pmc
+    runs of this benchmark, scaled by a plain copy kernel); {} when the summary
+    is missing or describes another system size."""
+    out = {}
+    if natoms != 2048000 or not os.path.exists(TRAFFIC_FILE):
+        return out
+    with open(TRAFFIC_FILE) as f:
+        for line in f:
+            t = line.split()
+            if len(t) >= 3 and t[0] == "traffic":
+                out[t[1]] = float(t[2])
+    return out
+
+
 def host_cpu_info():
     """CPU model (lscpu / /proc/cpuinfo) and online core count of this host."""
     import subprocess
@@ -449,6 +468,9 @@ def main():
         "atom_steps_per_s": N / (both_ms * 1e-3),
     }
     force_prl["frac"] = force_prl["achieved_GBs"] / HBM_PEAK_GBS
+    traffic = counter_traffic(N)
+    for k, v in sweeps.items():
+        v["traffic_bytes_per_launch"] = traffic.get(k)
     dom = max(sweeps, key=lambda k: sweeps[k]["avg_launch_ms"])
     achieved = sweeps[dom]["achieved_GBs"]
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rep.items()}
@@ -484,7 +506,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic.get(dom),
+            "traffic_source": (os.path.relpath(TRAFFIC_FILE, ROOT) if traffic.get(dom) else None),
             "bytes_per_atom": sweeps[dom]["bytes_per_atom"],
             "design_bytes_per_atom": sweeps[dom]["design_bytes_per_atom"],
             "avg_launch_ms": sweeps[dom]["avg_launch_ms"],
