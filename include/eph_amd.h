@@ -195,8 +195,10 @@ void* eph_stream(eph_context* ctx);
  * systems, form 0 otherwise).  eph_autotune times every form of each sweep on
  * the resident atoms (after eph_dev_post_force or the harness's attach) and
  * keeps the fastest; forms[2] receives the choice (density, force) and
- * ms[2 * EPH_SWEEP_FORMS] the times (density forms 0..3, then force forms
- * 0..3; 0 for a form the device cannot run).  f is not changed. */
+ * ms[2 * EPH_SWEEP_FORMS] the times (density forms, then force forms; 0 for
+ * a form the device cannot run).  f is not changed.  Forms 4, 5, 6 are form 0
+ * held to 4, 3 or 2 blocks per CU (fewer neighbour records in flight per L2,
+ * less latency hidden). */
 /* Candidate list of the density sweep (no counterpart in the reference; the
  * results are the same bits either way).  In form 0 the rho loop walks the
  * entries of the full list that were within r_cutoff + delta when the list
@@ -207,7 +209,7 @@ void* eph_stream(eph_context* ctx);
  * atoms with fixed shifts between list uploads, as eph_set_ghost_map states. */
 int eph_set_candidate_margin(eph_context* ctx, double delta);
 
-#define EPH_SWEEP_FORMS 4
+#define EPH_SWEEP_FORMS 7
 int eph_set_sweep_forms(eph_context* ctx, int density, int force);
 int eph_autotune(eph_context* ctx, int* forms, double* ms);
 

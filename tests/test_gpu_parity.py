@@ -108,7 +108,8 @@ def test_alloy_cascade_small(oracle, tmp_path, forms):
 
 
 @pytest.mark.parametrize("forms", [("auto", "auto"), ("chunk", "chunk"), ("chunk", "lds"),
-                                   ("lds", "chunk"), ("lds512", "lds256"), ("lds256", "lds512")])
+                                   ("lds", "chunk"), ("lds512", "lds256"), ("lds256", "lds512"),
+                                   ("chunk3", "chunk2"), ("chunk2", "chunk4")])
 def test_alloy_lds_table_path(oracle, tmp_path, forms):
     """fcc 21^3 (37,044 atoms), 4 species: large enough that the default is the
     persistent kernels staging all four rho(r^2) tables (128 KB) in LDS; every
@@ -130,7 +131,7 @@ def test_autotune_keeps_forces_and_bits(oracle, tmp_path):
     after = p.gpu.dev_atoms()
     for a, b in zip(before, after):
         assert np.array_equal(a, b)
-    forms = ("chunk", "lds", "lds512", "lds256")
+    forms = ("chunk", "lds", "lds512", "lds256", "chunk4", "chunk3", "chunk2")
     assert tuned["density"] in forms and tuned["force"] in forms
     assert set(tuned["density_ms"]) == set(forms) == set(tuned["force_ms"]), tuned
     assert all(v > 0 for v in tuned["density_ms"].values()), tuned

@@ -236,7 +236,7 @@ int eph_adjoint::step(long long n, Rec* r) {
   if ((rc = ctx->dev_post_force())) return fail(rc, ctx->err);
   if ((rc = ctx->dev_final_integrate())) return fail(rc, ctx->err);
   if (r) HCPY(r->TB.p, ctx->grid->T.p, G * sizeof(double));
-  if ((rc = ctx->dev_deposit())) return fail(rc, ctx->err);
+  if ((rc = ctx->dev_deposit(false, nullptr))) return fail(rc, ctx->err);
   if (r) HCPY(r->DT.p, ctx->grid->dT.p, G * sizeof(double));
   if (ctx->flags & EPH_FDM) {
     if ((rc = ctx->grid->solve(nullptr))) return fail(rc, ctx->grid->err);

@@ -520,9 +520,11 @@ int eph_context::autotune(int* forms, double* ms) {
   if (he == hipSuccess) he = hipEventCreate(&e1);
   constexpr int nf = kSweepForms;
   double t[2 * nf] = {};
-  const int nv = lds_tables_possible(tabs) ? nf : 1;  // forms 1.. need the tables in LDS
+  const bool lds_ok = lds_tables_possible(tabs);  // forms 1..3 need the tables in LDS
+  auto runs = [&](int v) { return lds_ok || v == 0 || v >= kChunkOcc4; };
   for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
-    for (int v = 0; v < nv && he == hipSuccess; ++v) {
+    for (int v = 0; v < nf && he == hipSuccess; ++v) {
+      if (!runs(v)) continue;
       for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // the first launch warms up
         he = hipEventRecord(e0, stream);
         if (he == hipSuccess)
@@ -541,8 +543,8 @@ int eph_context::autotune(int* forms, double* ms) {
   if (he != hipSuccess) return fail_hip(he, "autotune");
   auto best = [&](const double* tt) {
     int b = 0;
-    for (int v = 1; v < nv; ++v)
-      if (tt[v] < tt[b]) b = v;
+    for (int v = 1; v < nf; ++v)
+      if (runs(v) && tt[v] < tt[b]) b = v;
     return b;
   };
   form_density = best(t);
@@ -552,10 +554,35 @@ int eph_context::autotune(int* forms, double* ms) {
   return EPH_OK;
 }
 
+// final_integrate then end_of_step with nothing in between (the harness's
+// own Verlet step): where the bucketed deposit runs, one pass does both
+int eph_context::dev_final_integrate_end_of_step(long long ntimestep) {
+  if (atomic()) {
+    int rc = dev_final_integrate();
+    return rc != EPH_OK ? rc : dev_end_of_step(ntimestep);
+  }
+  int rc;
+  if (bucket_deposit() && !(flags & EPH_NOINT)) {
+    bool kicked = false;
+    if ((rc = dev_deposit(true, &kicked)) != EPH_OK) return rc;
+    if (!kicked) return fail(EPH_ERR_STATE, "the fused final kick did not run");
+  } else {
+    if ((rc = dev_final_integrate()) != EPH_OK) return rc;
+    if ((rc = dev_deposit(false, nullptr)) != EPH_OK) return rc;
+  }
+  return end_of_step_after_deposit(ntimestep);
+}
+
 int eph_context::dev_end_of_step(long long ntimestep) {
   if (atomic()) return eph::atomic_end_of_step(this);
-  int rc = dev_deposit();
+  int rc = dev_deposit(false, nullptr);
   if (rc != EPH_OK) return rc;
+  return end_of_step_after_deposit(ntimestep);
+}
+
+// the FDM solve and the heat-map output of end_of_step (fix_eph.cpp:392-399)
+int eph_context::end_of_step_after_deposit(long long ntimestep) {
+  int rc = EPH_OK;
   hipEvent_t e;
   if (flags & EPH_FDM) {
     timed_begin("fdm_solve", &e);
@@ -581,15 +608,27 @@ int eph_grid::bucket_bytes_ok(size_t G) {
   return EPH_OK;
 }
 
-// the deposit half of end_of_step (fix_eph.cpp:364-404): dT_e and the tally
-int eph_context::dev_deposit() {
+// the deposit goes through per-cell buckets when there is one and the grid
+// holds a few atoms per cell on average
+bool eph_context::bucket_deposit() const {
+  return (flags & (EPH_FRICTION | EPH_RANDOM)) &&
+         static_cast<double>(nlocal) <= 6.0 * static_cast<double>(grid->h.ntotal());
+}
+
+// the deposit half of end_of_step (fix_eph.cpp:364-404): dT_e and the tally.
+// kick: apply final_integrate's kick in the same pass where the bucketed
+// deposit runs (*kicked tells whether it did)
+int eph_context::dev_deposit(bool kick, bool* kicked_out) {
   if (!have_atoms) return fail(EPH_ERR_STATE, "no atoms");
+  bool kicked = false;
+  if (kicked_out) *kicked_out = false;
+  if (flags & EPH_NOINT) kick = false;
   const bool fric = flags & EPH_FRICTION, rnd = flags & EPH_RANDOM;
   const int nb = nblocks_of(nlocal);
   const size_t G = grid->h.ntotal();
   hipEvent_t e;
   // a few atoms per cell on average: per-cell buckets (three launches)
-  if ((fric || rnd) && static_cast<double>(nlocal) <= 6.0 * static_cast<double>(G)) {
+  if (bucket_deposit()) {
     EPH_HIP(this, grid->dep.reserve(nlocal, 0, nb));
     if (int rc = grid->bucket_bytes_ok(G)) return fail(rc, grid->err);
     DepositView d = grid->dep.view(true);
@@ -597,8 +636,10 @@ int eph_context::dev_deposit() {
     b.count = grid->bcount.p; b.items = grid->bitems.p; b.queue = grid->bqueue.p; b.nqueue = grid->bnq.p;
     timed_begin("deposit", &e);
     EPH_HIP(this, launch_deposit_bucket(atoms_view(), grid->v, step_params(), work_view(), d, b, fric, rnd,
-                                        scal.p, scal.p + 1, stream));
-    timed_end("deposit", e);
+                                        kick, scal.p, scal.p + 1, stream));
+    timed_end(kick ? "final_integrate+deposit" : "deposit", e);
+    kicked = kick;
+    if (kicked_out) *kicked_out = kicked;
     return EPH_OK;
   }
   if (fric || rnd) {

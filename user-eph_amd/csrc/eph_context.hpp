@@ -153,7 +153,10 @@ struct eph_context {
   int dev_post_force();
   int dev_final_integrate();
   int dev_end_of_step(long long ntimestep);
-  int dev_deposit();
+  int dev_final_integrate_end_of_step(long long ntimestep);
+  int end_of_step_after_deposit(long long ntimestep);
+  int dev_deposit(bool kick, bool* kicked);
+  bool bucket_deposit() const;
   int autotune(int* forms, double* ms);
   int grow_filter_state(size_t nall);
   int check_flags();

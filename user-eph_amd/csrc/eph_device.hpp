@@ -178,7 +178,11 @@ constexpr int kBigBlock = 1024;
 // kernel forms of the two neighbour sweeps: 0 reads the rho(r^2) tables
 // through the caches; the others stage them in LDS, with 1024, 512 or 256
 // threads per block (one block per CU)
-constexpr int kLdsForm1024 = 1, kLdsForm512 = 2, kLdsForm256 = 3, kSweepForms = 4;
+// kChunkOcc4/3/2: the cache-table form held to 4, 3 or 2 blocks per CU (its
+// LDS request set accordingly), which trades latency hiding for fewer
+// neighbour records in flight per L2
+constexpr int kLdsForm1024 = 1, kLdsForm512 = 2, kLdsForm256 = 3;
+constexpr int kChunkOcc4 = 4, kChunkOcc3 = 5, kChunkOcc2 = 6, kSweepForms = 7;
 // rho(r^2) tables up to this size are staged in LDS (160 KB per CU on gfx950)
 constexpr size_t kLdsTableMax = 128 * 1024;
 constexpr int kLdsMinAtoms = 32768;
@@ -226,12 +230,13 @@ hipError_t launch_deposit_points(int n, const double* x, const double* y, const 
 // kMaxSortInCell items.  d.count must hold the per-cell counts.
 hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_items, int* qbuf,
                                 hipStream_t s);
-// the whole deposit and tally through per-cell buckets (d.v2 may be null for
-// no random part, d.v1 for no friction part); b.count must be zero on entry
-// and is left zero
+// the whole deposit and tally through per-cell buckets (fric / rand: which
+// parts); with kick, the final velocity-Verlet kick of the group's atoms
+// first, in the same pass.  b.count must be zero on entry and is left zero
 hipError_t launch_deposit_bucket(const AtomsView& a, const GridView& g, const StepParams& p,
                                  const WorkView& w, const DepositView& d, const BucketView& b,
-                                 bool fric, bool rand, double* E_local, double* Ee, hipStream_t s);
+                                 bool fric, bool rand, bool kick, double* E_local, double* Ee,
+                                 hipStream_t s);
 hipError_t launch_energy_reduce(const DepositView& d, int nblocks, double* E_local, double* Ee,
                                 hipStream_t s);
 // grid

@@ -461,7 +461,9 @@ __global__ void k_cand_flags(int* cstate, int moved_cap) {
 template <int XI, bool FRIC, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_density_w(AtomsView a, TablesView t, StepParams p,
                                                       WorkView w) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStageW];
+  // (kBlock / 64) * kCoopStageW units; a larger request only lowers the
+  // blocks per CU
+  extern __shared__ double2 s_stage[];
   const int base = xcd_block() * kBlock;
   if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
   const int i = base + threadIdx.x;
@@ -781,7 +783,9 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
 template <bool FRIC, bool RAND, bool MULTI>
 __global__ __launch_bounds__(kBlock) void k_prl_forces(AtomsView a, TablesView t, GridView g,
                                                        StepParams p, WorkView w) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage * kCoopF];
+  // (kBlock / 64) * kCoopStage * kCoopF units; a larger request only lowers
+  // the blocks per CU
+  extern __shared__ double2 s_stage[];
   const int base = xcd_block() * kBlock;
   if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
   const int i = base + threadIdx.x;
@@ -1258,7 +1262,44 @@ __global__ __launch_bounds__(1024) void k_energy_reduce(const double* partial, i
 // the serial order of fix_eph.cpp:364-390.  A cell that receives more atoms
 // than its bucket holds is queued and summed by k_deposit_finish from an
 // ordered scan of every atom's cell.  Three launches in all.
+// one atom of the bucketed deposit at end-of-step velocity v: its energies
+// into the bucket of its cell and the values the cell sums read
 template <bool FRIC, bool RAND>
+__device__ __forceinline__ void deposit_bucket_atom(const int i, const double4& v, const AtomsView& a,
+                                                    const GridView& g, const StepParams& p,
+                                                    const WorkView& w, const DepositView& d,
+                                                    const BucketView& b, double* eF, double* eR) {
+  const double4 x = a.pos[i];
+  const int c = cell_of(g, x.x, x.y, x.z);
+  const int fs = a.fstride;
+  const double prescale = g.dV * p.dt;
+  double v1 = 0.0, v2 = 0.0;
+  if (FRIC) {
+    double dE = 0.0;
+    dE -= w.fE[i] * v.x * p.dt;
+    dE -= w.fE[fs + i] * v.y * p.dt;
+    dE -= w.fE[2 * fs + i] * v.z * p.dt;
+    *eF = dE;
+    v1 = dE / prescale;
+  }
+  if (RAND) {
+    double dE = 0.0;
+    dE -= w.fR[i] * v.x * p.dt;
+    dE -= w.fR[fs + i] * v.y * p.dt;
+    dE -= w.fR[2 * fs + i] * v.z * p.dt;
+    *eR = dE;
+    v2 = dE / prescale;
+  }
+  const int slot = atomicAdd(&b.count[c], 1);
+  if (slot < kBucket) b.items[static_cast<size_t>(c) * kBucket + slot] = i;
+  if (FRIC) d.v1[i] = v1;
+  if (RAND) d.v2[i] = v2;
+}
+
+// the per-atom pass; with KICK it first applies the final velocity-Verlet
+// kick (k_integrate<false>'s expressions), so that one pass serves
+// final_integrate and the deposit of end_of_step when nothing runs between
+template <bool FRIC, bool RAND, bool KICK>
 __global__ __launch_bounds__(kBlock) void k_deposit_bucket(AtomsView a, GridView g, StepParams p,
                                                            WorkView w, DepositView d,
                                                            BucketView b) {
@@ -1266,37 +1307,17 @@ __global__ __launch_bounds__(kBlock) void k_deposit_bucket(AtomsView a, GridView
   const int i = blockIdx.x * kBlock + threadIdx.x;
   if (i == 0) *b.nqueue = 0;  // the queue of this step starts empty
   double eF = 0.0, eR = 0.0;
-  if (i < a.nlocal) {
-    int c = -1;
-    double v1 = 0.0, v2 = 0.0;
-    if (a.mask[i] & a.groupbit) {
-      const double4 x = a.pos[i];
-      const double4 v = a.vel[i];
-      c = cell_of(g, x.x, x.y, x.z);
+  if (i < a.nlocal && (a.mask[i] & a.groupbit)) {
+    double4 v = a.vel[i];
+    if (KICK) {
       const int fs = a.fstride;
-      const double prescale = g.dV * p.dt;
-      if (FRIC) {
-        double dE = 0.0;
-        dE -= w.fE[i] * v.x * p.dt;
-        dE -= w.fE[fs + i] * v.y * p.dt;
-        dE -= w.fE[2 * fs + i] * v.z * p.dt;
-        eF = dE;
-        v1 = dE / prescale;
-      }
-      if (RAND) {
-        double dE = 0.0;
-        dE -= w.fR[i] * v.x * p.dt;
-        dE -= w.fR[fs + i] * v.y * p.dt;
-        dE -= w.fR[2 * fs + i] * v.z * p.dt;
-        eR = dE;
-        v2 = dE / prescale;
-      }
-      const int slot = atomicAdd(&b.count[c], 1);
-      if (slot < kBucket) b.items[static_cast<size_t>(c) * kBucket + slot] = i;
+      const double dtfm = p.dtf / p.mass[a.type[i]];
+      v.x += dtfm * a.f[i];
+      v.y += dtfm * a.f[fs + i];
+      v.z += dtfm * a.f[2 * fs + i];
+      a.vel[i] = v;
     }
-    d.cell[i] = c;
-    d.v1[i] = v1;
-    d.v2[i] = v2;
+    deposit_bucket_atom<FRIC, RAND>(i, v, a, g, p, w, d, b, &eF, &eR);
   }
   const double sF = block_sum(eF, sh);
   const double sR = block_sum(eR, sh);
@@ -1361,9 +1382,10 @@ __global__ __launch_bounds__(kBlock) void k_deposit_bucket_cells(GridView g, Dep
 // blocks 1..: the queued cells, each by an ordered scan over all atoms:
 // every wave ballots the atoms of the cell among its 64, and thread 0 adds
 // their values in ascending order
-__global__ __launch_bounds__(1024) void k_deposit_finish(GridView g, DepositView d, BucketView b,
-                                                         int n_items, int nblocks, double* E_local,
+__global__ __launch_bounds__(1024) void k_deposit_finish(AtomsView a, GridView g, DepositView d,
+                                                         BucketView b, int nblocks, double* E_local,
                                                          double* Ee) {
+  const int n_items = a.nlocal;
   __shared__ double sh[1024];
   __shared__ unsigned long long hits[16];
   if (blockIdx.x == 0) {
@@ -1380,7 +1402,12 @@ __global__ __launch_bounds__(1024) void k_deposit_finish(GridView g, DepositView
       if (!v) continue;
       for (int base = 0; base < n_items; base += 1024) {
         const int i = base + threadIdx.x;
-        const unsigned long long m = __ballot(i < n_items && d.cell[i] == c);
+        bool in = false;
+        if (i < n_items && (a.mask[i] & a.groupbit)) {
+          const double4 x = a.pos[i];
+          in = cell_of(g, x.x, x.y, x.z) == c;
+        }
+        const unsigned long long m = __ballot(in);
         if (lane == 0) hits[wv] = m;
         __syncthreads();
         if (threadIdx.x == 0)
@@ -1929,10 +1956,21 @@ bool lds_tables_possible(const TablesView& t) { return lds_tables_fit(t); }
 // form that reads the tables through the caches.  -1 picks by system size,
 // with `dflt` for large systems.
 static int resolve_form(const AtomsView& a, const TablesView& t, int form, int dflt) {
-  if (form == 0 || !lds_tables_fit(t)) return 0;
+  if (form == 0 || form >= kChunkOcc4) return form;  // no tables in LDS
+  if (!lds_tables_fit(t)) return 0;
   if (form < 0) return use_lds_tables(a, t) ? dflt : 0;
   return form;
 }
+
+// LDS request of a cache-table launch: the staging its waves use, or more
+// so that at most 4, 3 or 2 blocks share a CU (160 KB of LDS per CU)
+static size_t chunk_lds(int form, size_t need) {
+  const size_t want = form == kChunkOcc4 ? 38912 : form == kChunkOcc3 ? 49152
+                    : form == kChunkOcc2 ? 65536 : 0;
+  return want > need ? want : need;
+}
+constexpr size_t kDensityStageBytes = (kBlock / 64) * kCoopStageW * sizeof(double2);
+constexpr size_t kForceStageBytes = (kBlock / 64) * kCoopStage * kCoopF * sizeof(double2);
 
 template <int XI, bool F, bool M>
 static void density_lds(int blk, size_t lds, const AtomsView& a, const TablesView& t,
@@ -1954,7 +1992,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
   const int f = resolve_form(a, t, form, 0);
-  if (f != 0) {
+  if (f >= kLdsForm1024 && f <= kLdsForm256) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int blk = lds_block(f);
 #define EPH_DWL_M(XI, F) if (multi) density_lds<XI, F, true>(blk, lds, a, t, p, w, s); \
@@ -1974,7 +2012,8 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   WorkView wc = w;
   if (!keep) wc.rcand2 = 0.0;
   if (keep) k_cand_check<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal + a.nghost, a.pos, wc);
-#define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, 0, s>>>(a, t, p, wc)
+  const size_t dl = chunk_lds(f, kDensityStageBytes);
+#define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, dl, s>>>(a, t, p, wc)
 #define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
 #define EPH_DW_F(XI) if (fric) { EPH_DW_M(XI, true); } else { EPH_DW_M(XI, false); }
   switch (p.xi_mode) {
@@ -2020,7 +2059,7 @@ hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const Grid
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (!fric && !rnd) return hipSuccess;
   const int f = resolve_form(a, t, form, 0);
-  if (f != 0) {
+  if (f >= kLdsForm1024 && f <= kLdsForm256) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int blk = lds_block(f);
 #define EPH_PFL_M(F, R) if (multi) forces_lds<F, R, true>(blk, lds, a, t, g, p, w, s); \
@@ -2032,7 +2071,8 @@ hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const Grid
     return hipGetLastError();
   }
   const int gr = nblk(a.nlocal);
-#define EPH_PF(F, R, M) k_prl_forces<F, R, M><<<gr, kBlock, 0, s>>>(a, t, g, p, w)
+  const size_t fl = chunk_lds(f, kForceStageBytes);
+#define EPH_PF(F, R, M) k_prl_forces<F, R, M><<<gr, kBlock, fl, s>>>(a, t, g, p, w)
 #define EPH_PF_M(F, R) if (multi) EPH_PF(F, R, true); else EPH_PF(F, R, false)
   if (fric && rnd) { EPH_PF_M(true, true); }
   else if (fric) { EPH_PF_M(true, false); }
@@ -2134,13 +2174,17 @@ hipError_t launch_deposit_cells(const GridView& g, const DepositView& d, int n_i
 
 hipError_t launch_deposit_bucket(const AtomsView& a, const GridView& g, const StepParams& p,
                                  const WorkView& w, const DepositView& d, const BucketView& b,
-                                 bool fric, bool rand, double* E_local, double* Ee, hipStream_t s) {
+                                 bool fric, bool rand, bool kick, double* E_local, double* Ee,
+                                 hipStream_t s) {
   const int G = g.nx * g.ny * g.nz;
   const int nb = nblk(a.nlocal);
   if (a.nlocal > 0) {
-    if (fric && rand) k_deposit_bucket<true, true><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
-    else if (fric) k_deposit_bucket<true, false><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
-    else k_deposit_bucket<false, true><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b);
+#define EPH_DB(F, R) if (kick) k_deposit_bucket<F, R, true><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b); \
+                     else k_deposit_bucket<F, R, false><<<nb, kBlock, 0, s>>>(a, g, p, w, d, b)
+    if (fric && rand) { EPH_DB(true, true); }
+    else if (fric) { EPH_DB(true, false); }
+    else { EPH_DB(false, true); }
+#undef EPH_DB
   } else {
     k_fill_int<<<1, 64, 0, s>>>(b.nqueue, 1, 0);
   }
@@ -2148,7 +2192,7 @@ hipError_t launch_deposit_bucket(const AtomsView& a, const GridView& g, const St
   if (!fric) dc.v1 = nullptr;
   if (!rand) dc.v2 = nullptr;
   k_deposit_bucket_cells<<<nblk(G), kBlock, 0, s>>>(g, dc, b, G);
-  k_deposit_finish<<<1 + 16, 1024, 0, s>>>(g, dc, b, a.nlocal, a.nlocal > 0 ? nb : 0, E_local, Ee);
+  k_deposit_finish<<<1 + 16, 1024, 0, s>>>(a, g, dc, b, a.nlocal > 0 ? nb : 0, E_local, Ee);
   return hipGetLastError();
 }
 

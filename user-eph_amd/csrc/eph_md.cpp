@@ -610,8 +610,8 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
                           md->cfg.lj_sigma, md->cfg.cut_lj * md->cfg.cut_lj, s));
     EPH_HIP(md, hipEventRecord(ev[3], s));
     if ((rc = ctx->dev_post_force())) break;
-    if ((rc = ctx->dev_final_integrate())) break;
-    if ((rc = ctx->dev_end_of_step(step))) break;
+    // nothing runs between the two hooks here: one pass where the deposit allows
+    if ((rc = ctx->dev_final_integrate_end_of_step(step))) break;
     EPH_HIP(md, hipEventRecord(ev[4], s));
     EPH_HIP(md, hipEventSynchronize(ev[4]));
     float a = 0, b = 0, c = 0, d = 0;

@@ -406,9 +406,11 @@ class FixEPH:
         self._check(lib().eph_set_filter_state(self.h, which, _d(a)))
 
     # kernel forms of the sweeps: tables through the caches, or in LDS with
-    # 1024 / 512 / 256 threads per block
-    _FORM_NAMES = ("chunk", "lds", "lds512", "lds256")
-    _FORMS = {"chunk": 0, "lds": 1, "lds512": 2, "lds256": 3, "auto": -1}
+    # 1024 / 512 / 256 threads per block; chunk4/3/2: tables through the
+    # caches with at most 4, 3 or 2 blocks per CU
+    _FORM_NAMES = ("chunk", "lds", "lds512", "lds256", "chunk4", "chunk3", "chunk2")
+    _FORMS = {"chunk": 0, "lds": 1, "lds512": 2, "lds256": 3, "chunk4": 4, "chunk3": 5,
+              "chunk2": 6, "auto": -1}
 
     def set_sweep_forms(self, density="auto", force="auto"):
         """kernel forms of the density and force sweeps: 'chunk', 'lds' or 'auto'"""
