@@ -200,13 +200,16 @@ void* eph_stream(eph_context* ctx);
  * held to 4, 3 or 2 blocks per CU (fewer neighbour records in flight per L2,
  * less latency hidden). */
 /* Candidate list of the density sweep (no counterpart in the reference; the
- * results are the same bits either way).  In form 0 the rho loop walks the
- * entries of the full list that were within r_cutoff + delta when the list
- * was last scanned, as long as no local atom has moved 0.45 delta since; the
- * entries it skips are then beyond r_cutoff and inactive.  Otherwise it scans
- * the full list and retakes the candidates.  delta < 0: 0.1 r_cutoff (the
- * default), 0: always the full list.  Ghost atoms must be images of local
- * atoms with fixed shifts between list uploads, as eph_set_ghost_map states. */
+ * results are the same bits either way).  In form 0 (and 4-6) the rho loop of a
+ * wave walks the entries of the full list that were within r_cutoff + delta
+ * when the candidates were last taken, unless an atom (local or ghost) that
+ * has moved 0.45 delta since then lies within r_cutoff + 0.1 delta of the
+ * wave's atoms; such waves scan the full list.  Any change of the lists,
+ * ghost map, atoms, types, masks or group retakes the candidates of all atoms
+ * in the next sweep, and so do more than 512 moved atoms.  Ghost positions
+ * uploaded at every hook are checked as they are (a box that changes between
+ * list uploads is allowed).  delta < 0: 0.1 r_cutoff (the default), 0: always
+ * the full list. */
 int eph_set_candidate_margin(eph_context* ctx, double delta);
 
 #define EPH_SWEEP_FORMS 7
