@@ -136,6 +136,29 @@ int eph_initial_integrate(eph_context* ctx, double* x, double* v, const double* 
  * force_prl :687-837); x, v are the current [nall][3] arrays, f [nlocal][3]
  * is updated in place (f += f_EPH, f += f_RNG per the flags). */
 int eph_post_force(eph_context* ctx, const double* x, const double* v, double* f);
+/* post_force of a run over several MPI ranks, split where FixEPH exchanges
+ * ghost values (comm->forward_comm with states XI, RHO, WI, fix_eph.cpp:863-871
+ * and pack/unpack_forward_comm :951-1009).  Those exchanges feed only force_prl's
+ * stages 2 and 3, so one exchange of EPH_HALO_COLS values per atom carries all
+ * three: eph_post_force_local draws xi and computes rho, alpha and w of the local
+ * atoms and writes halo[nlocal][EPH_HALO_COLS] = (xi_x, xi_y, xi_z, rho, w_x,
+ * w_y, w_z); the caller sends the rows of the owners to their ghosts, and
+ * eph_post_force_ghosts takes halo_ghost[nghost][EPH_HALO_COLS] in ghost order,
+ * finishes the forces and updates f[nlocal][3].  x, v are [nall][3].  Styles eph,
+ * eph/coloured and eph/coloured/exp with model 4 (else EPH_ERR_UNSUPPORTED). */
+#define EPH_HALO_COLS 7
+int eph_post_force_local(eph_context* ctx, const double* x, const double* v, double* halo);
+int eph_post_force_ghosts(eph_context* ctx, const double* halo_ghost, double* f);
+/* end_of_step of a run over several ranks, split at EPH_FDM's sync_before
+ * (eph_fdm.h:479-491) and the tally's MPI_Allreduce (fix_eph.cpp:402): the first
+ * call deposits this rank's atoms and returns the grid's dT_e[G] and this rank's
+ * E_local; the caller sums both over the ranks and passes the sums to the second,
+ * which solves the grid (every rank the same bits, so no broadcast is needed)
+ * and adds the energy to Ee.  write_output: write the heat map (one rank). */
+int eph_end_of_step_local(eph_context* ctx, const double* x, const double* v, double* dT,
+                          double* E_local);
+int eph_end_of_step_reduced(eph_context* ctx, const double* dT, double E, long long ntimestep,
+                            int write_output);
 /* FixEPH::final_integrate, fix_eph.cpp:330-348 */
 int eph_final_integrate(eph_context* ctx, double* v, const double* f);
 /* FixEPH::end_of_step, fix_eph.cpp:350-429 (+ EPH_FDM::solve, eph_fdm.h:267-400) */

@@ -24,6 +24,7 @@ class Fix : protected Pointers {
   int groupbit = 1;
   int vector_flag = 0, size_vector = 0, global_freq = 0, extvector = 0;
   int peratom_flag = 0, size_peratom_cols = 0, peratom_freq = 0;
+  int comm_forward = 0;  // values per atom of pack/unpack_forward_comm
   double **array_atom = nullptr;
   Fix(LAMMPS *lmp, int narg, char **arg) : Pointers(lmp) {
     id_store = narg > 0 ? arg[0] : "";
@@ -43,6 +44,8 @@ class Fix : protected Pointers {
   virtual void grow_arrays(int) {}
   virtual void pre_exchange() {}
   virtual void copy_arrays(int, int, int) {}
+  virtual int pack_forward_comm(int, int *, double *, int, int *) { return 0; }
+  virtual void unpack_forward_comm(int, int, double *) {}
 
  private:
   std::string id_store;

@@ -1,5 +1,6 @@
 // Test harness: the LAMMPS object and the pointers every class sees.
 #pragma once
+#include <mpi.h>
 #include "lmptype.h"
 namespace LAMMPS_NS {
 class Atom;
@@ -12,6 +13,7 @@ class Neighbor;
 class Update;
 class LAMMPS {
  public:
+  MPI_Comm world = MPI_COMM_WORLD;
   Atom *atom = nullptr;
   Comm *comm = nullptr;
   Domain *domain = nullptr;
@@ -24,12 +26,14 @@ class LAMMPS {
 class Pointers {
  public:
   explicit Pointers(LAMMPS *ptr)
-      : lmp(ptr), atom(ptr->atom), comm(ptr->comm), domain(ptr->domain), error(ptr->error),
-        force(ptr->force), memory(ptr->memory), neighbor(ptr->neighbor), update(ptr->update) {}
+      : lmp(ptr), world(ptr->world), atom(ptr->atom), comm(ptr->comm), domain(ptr->domain),
+        error(ptr->error), force(ptr->force), memory(ptr->memory), neighbor(ptr->neighbor),
+        update(ptr->update) {}
   virtual ~Pointers() = default;
 
  protected:
   LAMMPS *lmp;
+  MPI_Comm &world;
   Atom *atom;
   Comm *comm;
   Domain *domain;

@@ -265,12 +265,16 @@ def test_lammps_adaptor_compiles_against_the_lammps_api():
     the harness driver too"""
     if shutil.which("g++") is None:
         pytest.skip("no g++")
+    mpi_inc = "/opt/conda/include"  # MPICH's headers (the adaptor calls MPI_Allreduce)
+    if not os.path.exists(os.path.join(mpi_inc, "mpi.h")):
+        pytest.skip("no mpi.h")
     root = ROOT
     mock = os.path.join(ROOT, "tests", "lammps_mock")
     for src in (os.path.join(root, "user-eph_amd", "lammps", "fix_eph_amd.cpp"),
-                os.path.join(mock, "mock_driver.cpp")):
+                os.path.join(mock, "mock_driver.cpp"), os.path.join(mock, "mock_mpi_driver.cpp")):
         r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-Werror",
                             "-I" + mock, "-I" + os.path.join(root, "include"),
-                            "-I" + os.path.join(root, "user-eph_amd", "lammps"), src],
+                            "-I" + os.path.join(root, "user-eph_amd", "lammps"),
+                            "-I" + mpi_inc, src],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr

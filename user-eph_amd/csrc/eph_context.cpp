@@ -278,7 +278,7 @@ int eph_context::reserve_atoms(int nl, int ng) {
   EPH_HIP(this, cref.reserve(nall));
   EPH_HIP(this, moved.reserve(kMovedCap));
   EPH_HIP(this, cstate.reserve(8));
-  EPH_HIP(this, stage.reserve(3 * nall));
+  EPH_HIP(this, stage.reserve(static_cast<size_t>(EPH_HALO_COLS) * nall));  // rows, halos
   EPH_HIP(this, xi_in.reserve(3 * (size_t)st));
   if (st != stride || f.n < 3 * (size_t)st) {
     EPH_HIP(this, f.reserve(3 * (size_t)st));
@@ -500,6 +500,46 @@ int eph_context::dev_post_force() {
   return EPH_OK;
 }
 
+// post_force of a multi-rank run, first part: the noise, rho, alpha and w of
+// the local atoms (fix_eph.cpp:852-862, 866-867 and force_prl's stage 1);
+// their xi, rho and w then go to the other ranks' ghosts (the XI, RHO and WI
+// exchanges of :863-871, which only the later stages read)
+int eph_context::dev_post_force_local() {
+  if (!have_atoms || !have_list) return fail(EPH_ERR_STATE, "atoms and neighbour list must be set");
+  if (atomic() || colour == 3 || model != 4)
+    return fail(EPH_ERR_UNSUPPORTED, "the split post_force serves the PRL model of eph, "
+                                     "eph/coloured and eph/coloured/exp");
+  StepParams p = step_params();
+  p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
+  const AtomsView a = atoms_view();
+  const WorkView w = work_view();
+  hipEvent_t e;
+  if (!pv_fresh || !a.pv) {
+    EPH_HIP(this, launch_pack_pv(a, w, stream));
+    pv_fresh = a.pv != nullptr;
+  }
+  timed_begin("density_w", &e);
+  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
+  timed_end("density_w", e);
+  return EPH_OK;
+}
+
+// the second part, once the ghosts' records hold their exchanged values:
+// stages 2 and 3 and the force update
+int eph_context::dev_post_force_ghosts() {
+  StepParams p = step_params();
+  p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
+  const AtomsView a = atoms_view();
+  const WorkView w = work_view();
+  hipEvent_t e;
+  timed_begin("prl_forces", &e);
+  EPH_HIP(this, launch_prl_forces(a, tabs, grid->v, p, w, stream, form_force));
+  timed_end("prl_forces", e);
+  xi_mode_next = 0;
+  have_forces = true;
+  return EPH_OK;
+}
+
 // Times every form of each sweep on the resident atoms and keeps the fastest.
 // The density sweep rewrites only what the next post_force rewrites anyway;
 // the force sweep runs with both force additions off, so f is not touched.
@@ -590,7 +630,7 @@ int eph_context::end_of_step_after_deposit(long long ntimestep) {
     if (rc != EPH_OK) return fail(rc, grid->err);
     timed_end("fdm_solve", e);
   }
-  if (T_freq > 0 && (ntimestep % T_freq) == 0) {
+  if (write_output && T_freq > 0 && (ntimestep % T_freq) == 0) {
     std::vector<double> T;
     rc = grid->download(0, T);
     if (rc != EPH_OK) return fail(rc, grid->err);
@@ -636,7 +676,7 @@ int eph_context::dev_deposit(bool kick, bool* kicked_out) {
     b.count = grid->bcount.p; b.items = grid->bitems.p; b.queue = grid->bqueue.p; b.nqueue = grid->bnq.p;
     timed_begin("deposit", &e);
     EPH_HIP(this, launch_deposit_bucket(atoms_view(), grid->v, step_params(), work_view(), d, b, fric, rnd,
-                                        kick, scal.p, scal.p + 1, stream));
+                                        kick, scal.p, scal.p + (defer_tally ? 2 : 1), stream));
     timed_end(kick ? "final_integrate+deposit" : "deposit", e);
     kicked = kick;
     if (kicked_out) *kicked_out = kicked;
@@ -652,7 +692,7 @@ int eph_context::dev_deposit(bool kick, bool* kicked_out) {
     if (!fric) dc.v1 = nullptr;
     if (!rnd) dc.v2 = nullptr;
     EPH_HIP(this, launch_deposit_cells(grid->v, dc, nlocal, grid->dep.qbuf.p, stream));
-    EPH_HIP(this, launch_energy_reduce(d, nb, scal.p, scal.p + 1, stream));
+    EPH_HIP(this, launch_energy_reduce(d, nb, scal.p, scal.p + (defer_tally ? 2 : 1), stream));
     timed_end("deposit", e);
   } else {
     EPH_HIP(this, launch_fill_double(scal.p, 1, 0.0, stream));
@@ -1074,6 +1114,86 @@ int eph_end_of_step(eph_context* ctx, const double* x, const double* v, long lon
   if ((rc = upload_rows(ctx, x, ctx->nlocal, ctx->pos.p))) return rc;
   if ((rc = upload_rows(ctx, v, ctx->nlocal, ctx->vel.p))) return rc;
   if ((rc = ctx->dev_end_of_step(ntimestep))) return rc;
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx->check_flags();
+}
+
+int eph_post_force_local(eph_context* ctx, const double* x, const double* v, double* halo) {
+  if (!ctx || !x || !v || (!halo && ctx->nlocal > 0)) return EPH_ERR_ARGS;
+  if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
+  const int nall = ctx->nlocal + ctx->nghost;
+  int rc;
+  if ((rc = upload_rows(ctx, x, nall, ctx->pos.p))) return rc;
+  if ((rc = upload_rows(ctx, v, nall, ctx->vel.p))) return rc;
+  if ((rc = ctx->dev_post_force_local())) return rc;
+  if (ctx->nlocal) {
+    EPH_HIP(ctx, launch_halo_pack(ctx->rec.p, ctx->nlocal, ctx->stage.p, ctx->stream));
+    if ((rc = ctx->d2h(halo, ctx->stage.p, (size_t)EPH_HALO_COLS * ctx->nlocal * sizeof(double)))) return rc;
+  }
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx->check_flags();
+}
+
+int eph_post_force_ghosts(eph_context* ctx, const double* halo_ghost, double* f) {
+  if (!ctx || !f || (!halo_ghost && ctx->nghost > 0)) return EPH_ERR_ARGS;
+  if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
+  int rc;
+  if (ctx->nghost) {
+    // the ghosts' records from their exchanged xi, rho, w, with alpha from rho
+    // as fix_eph.cpp:757 and :800 evaluate it for a neighbour; enqueued before
+    // the force upload reuses the staging buffer
+    if ((rc = ctx->h2d(ctx->stage.p, halo_ghost, (size_t)EPH_HALO_COLS * ctx->nghost * sizeof(double))))
+      return rc;
+    EPH_HIP(ctx, launch_ghost_halo(ctx->atoms_view(), ctx->tabs, ctx->work_view(), ctx->stage.p,
+                                   ctx->stream));
+  }
+  if ((rc = upload_force(ctx, f))) return rc;
+  if ((rc = ctx->dev_post_force_ghosts())) return rc;
+  if (ctx->nlocal) {
+    EPH_HIP(ctx, launch_soa_into_rows(ctx->f.p, ctx->nlocal, ctx->fstride, ctx->stage.p, ctx->stream));
+    if ((rc = ctx->d2h(f, ctx->stage.p, 3 * (size_t)ctx->nlocal * sizeof(double)))) return rc;
+  }
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return ctx->check_flags();
+}
+
+int eph_end_of_step_local(eph_context* ctx, const double* x, const double* v, double* dT,
+                          double* E_local) {
+  if (!ctx || !x || !v || !dT || !E_local) return EPH_ERR_ARGS;
+  if (!ctx->have_atoms) return ctx->fail(EPH_ERR_STATE, "no atoms");
+  if (ctx->atomic()) return ctx->fail(EPH_ERR_UNSUPPORTED, "eph/atomic has no T_e grid");
+  int rc;
+  if ((rc = upload_rows(ctx, x, ctx->nlocal, ctx->pos.p))) return rc;
+  if ((rc = upload_rows(ctx, v, ctx->nlocal, ctx->vel.p))) return rc;
+  ctx->defer_tally = true;
+  rc = ctx->dev_deposit(false, nullptr);
+  ctx->defer_tally = false;
+  if (rc) return rc;
+  std::vector<double> d;
+  if ((rc = ctx->grid->download(1, d))) return ctx->fail(rc, ctx->grid->err);
+  std::copy(d.begin(), d.end(), dT);
+  EPH_HIP(ctx, hipMemcpyAsync(E_local, ctx->scal.p, sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  return EPH_OK;
+}
+
+int eph_end_of_step_reduced(eph_context* ctx, const double* dT, double E, long long ntimestep,
+                            int write_output) {
+  if (!ctx || !dT) return EPH_ERR_ARGS;
+  if (ctx->atomic()) return ctx->fail(EPH_ERR_UNSUPPORTED, "eph/atomic has no T_e grid");
+  eph_grid* g = ctx->grid;
+  const size_t G = g->h.ntotal();
+  double sc[2] = {0.0, 0.0};
+  EPH_HIP(ctx, hipMemcpyAsync(sc, ctx->scal.p, 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
+  sc[0] = E;
+  sc[1] += E;  // Ee += the sum over ranks (fix_eph.cpp:402-404)
+  EPH_HIP(ctx, hipMemcpyAsync(ctx->scal.p, sc, 2 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+  if (int rc = ctx->h2d(g->dT.p, dT, G * sizeof(double))) return rc;
+  ctx->write_output = write_output != 0;
+  const int rc = ctx->end_of_step_after_deposit(ntimestep);
+  ctx->write_output = true;
+  if (rc) return rc;
   EPH_HIP(ctx, hipStreamSynchronize(ctx->stream));
   return ctx->check_flags();
 }

@@ -120,6 +120,10 @@ struct eph_context {
   // that move atoms keep them so; anything else that writes pos or vel
   // (uploads, restores) clears this and the next post_force repacks
   bool pv_fresh = false;
+  // the multi-rank hooks: the deposit leaves Ee alone (the caller adds the
+  // sum over ranks), and only the rank that writes files writes the heat map
+  bool defer_tally = false, write_output = true;
+  std::vector<double> halo_host;  // [nall][7] staging of the halo hooks
   // kernel forms of the two sweeps (-1 chosen by system size; see eph_autotune)
   int form_density = -1, form_force = -1;
   long long step = 0;
@@ -151,6 +155,8 @@ struct eph_context {
   int set_neighbors(int nl, const int* off, const int* idx, int neighmask);
   int dev_initial_integrate();
   int dev_post_force();
+  int dev_post_force_local();   // post_force up to the ghost exchange
+  int dev_post_force_ghosts();  // the rest, with the ghosts' halo values in place
   int dev_final_integrate();
   int dev_end_of_step(long long ntimestep);
   int dev_final_integrate_end_of_step(long long ntimestep);

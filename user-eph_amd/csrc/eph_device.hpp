@@ -208,6 +208,10 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
 bool lds_tables_possible(const TablesView& t);
 hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s);
 hipError_t launch_ghost_rec(const AtomsView& a, const WorkView& w, hipStream_t s);
+// multi-rank halo: the locals' (xi, rho, w) rows out, the ghosts' rows in
+hipError_t launch_halo_pack(const ForceRec* rec, int n, double* out, hipStream_t s);
+hipError_t launch_ghost_halo(const AtomsView& a, const TablesView& t, const WorkView& w,
+                             const double* in, hipStream_t s);
 hipError_t launch_prl_forces(const AtomsView& a, const TablesView& t, const GridView& g,
                              const StepParams& p, const WorkView& w, hipStream_t s, int form = -1);
 hipError_t launch_integrate(const AtomsView& a, const StepParams& p, bool initial, hipStream_t s);

@@ -520,6 +520,36 @@ __device__ __forceinline__ void ghost_rec(const int g, const AtomsView& a, const
   r->xa = w.rec[o].xa;
 }
 
+// the halo values of the local atoms, [nlocal][7] = (xi, rho, w): what the
+// XI, RHO and WI forward exchanges of fix_eph.cpp:951-985 send
+__global__ __launch_bounds__(kBlock) void k_halo_pack(const ForceRec* __restrict__ rec, int n,
+                                                      double* __restrict__ out) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  if (i >= n) return;
+  const double4 rw = rec[i].rw, xa = rec[i].xa;
+  double* o = out + 7 * static_cast<size_t>(i);
+  o[0] = xa.x; o[1] = xa.y; o[2] = xa.z; o[3] = rw.w; o[4] = rw.x; o[5] = rw.y; o[6] = rw.z;
+}
+
+// a ghost's record from its exchanged (xi, rho, w), [nghost][7]; alpha from
+// rho as for a local atom (the same function, so the same bits as its owner's)
+template <bool MULTI>
+__global__ __launch_bounds__(kBlock) void k_ghost_halo(AtomsView a, TablesView t, WorkView w,
+                                                       const double* __restrict__ in) {
+  const int g = blockIdx.x * kBlock + threadIdx.x;
+  if (g >= a.nghost) return;
+  const int i = a.nlocal + g;
+  const double* h = in + 7 * static_cast<size_t>(g);
+  const double4 pi = a.pos[i];
+  const bool grp = (a.mask[i] & a.groupbit) != 0;
+  const double rho = h[3];
+  const double alpha = density_alpha<MULTI>(grp, rho, pi, t, w);
+  ForceRec* r = w.rec + i;
+  r->p = pi;
+  r->rw = make_double4(h[4], h[5], h[6], rho);
+  r->xa = make_double4(h[0], h[1], h[2], alpha);
+}
+
 __global__ __launch_bounds__(kBlock) void k_ghost_rec(AtomsView a, WorkView w) {
   const int g = blockIdx.x * kBlock + threadIdx.x;
   if (g < a.nghost) ghost_rec(g, a, w);
@@ -2032,6 +2062,20 @@ hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s) 
   const int n = a.nlocal + a.nghost;
   if (n == 0) return hipSuccess;
   k_pack_pv<<<nblk(n), kBlock, 0, s>>>(a, w, n);
+  return hipGetLastError();
+}
+
+hipError_t launch_halo_pack(const ForceRec* rec, int n, double* out, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  k_halo_pack<<<nblk(n), kBlock, 0, s>>>(rec, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_ghost_halo(const AtomsView& a, const TablesView& t, const WorkView& w,
+                             const double* in, hipStream_t s) {
+  if (a.nghost == 0) return hipSuccess;
+  if (t.n_elem > 1) k_ghost_halo<true><<<nblk(a.nghost), kBlock, 0, s>>>(a, t, w, in);
+  else k_ghost_halo<false><<<nblk(a.nghost), kBlock, 0, s>>>(a, t, w, in);
   return hipGetLastError();
 }
 

@@ -5,11 +5,19 @@
  * neighbour-list build, when LAMMPS may have exchanged, sorted or re-imaged
  * atoms.  Per-atom state of the stateful styles comes to LAMMPS's rows before
  * the exchange (pre_exchange), moves with the atoms (copy_arrays) and goes
- * back after the list build.  One MPI rank per GPU; the per-step ghost
- * exchange of rho, w and xi (fix_eph.cpp:612, 744, 864, 871) happens inside
- * the context, which holds every ghost's owner, so this adaptor serves
- * single-rank runs. */
+ * back after the list build.  On one rank the per-step ghost exchange of
+ * rho, w and xi (fix_eph.cpp:612, 744, 864, 871) happens inside the context,
+ * which holds every ghost's owner.  Over several ranks (each with its own
+ * context, ranks may share a GPU) post_force is split where FixEPH exchanges:
+ * the local atoms' xi, rho and w go to the ghosts through comm->forward_comm
+ * (one exchange serves the XI, RHO and WI states, which only force_prl's
+ * stages 2 and 3 read), and end_of_step is split at EPH_FDM's sync_before:
+ * the grid's deposits and the tally are summed with MPI_Allreduce
+ * (eph_fdm.h:479-491, fix_eph.cpp:402), after which every rank solves the
+ * same grid to the same bits (no broadcast needed). */
 #include "fix_eph_amd.h"
+
+#include <mpi.h>
 
 #include <algorithm>
 #include <cstring>
@@ -48,10 +56,9 @@ std::string reference_style(const char *s) {
 
 FixEPHAMD::FixEPHAMD(LAMMPS *lmp, int narg, char **arg) :
     Fix(lmp, narg, arg), ctx(nullptr), list(nullptr), ncols(8), array(nullptr), list_step(-1),
-    nstate(0), state(nullptr), state_on_host(false)
+    nstate(0), state(nullptr), state_on_host(false), multi(comm->nprocs > 1)
 {
   if (narg < 3) error->all(FLERR, "Illegal fix eph/amd command");
-  if (comm->nprocs > 1) error->all(FLERR, "fix eph/amd runs on one MPI rank");
   if (atom->map_style == Atom::MAP_NONE)
     error->all(FLERR, "fix eph/amd needs an atom map (atom_modify map array)");
   // the context parses the same words as fix eph, with the reference style
@@ -69,6 +76,17 @@ FixEPHAMD::FixEPHAMD(LAMMPS *lmp, int narg, char **arg) :
   if (rc != EPH_OK) error->all(FLERR, eph_last_error(nullptr));
   check(eph_set_groupbit(ctx, groupbit));
   check(eph_set_masses(ctx, atom->mass, atom->ntypes));
+  if (multi) {
+    int model = 0;
+    check(eph_get_params(ctx, nullptr, &model, nullptr, nullptr, nullptr, nullptr, nullptr));
+    if (model != 4 || (style != "eph" && style != "eph/coloured" && style != "eph/coloured/exp"))
+      error->all(FLERR, "fix eph/amd over several MPI ranks: styles eph, eph/coloured and "
+                        "eph/coloured/exp with model 4");
+    comm_forward = EPH_HALO_COLS;  // xi, rho, w per atom
+    int dims[3] = {0, 0, 0};
+    check(eph_grid_info(eph_context_grid(ctx), dims, nullptr, nullptr));
+    dT.resize(static_cast<size_t>(dims[0]) * dims[1] * dims[2]);
+  }
 
   vector_flag = 1;  // Ee and the mean T_e (fix_eph.cpp:406-428)
   size_vector = 2;
@@ -136,7 +154,37 @@ void FixEPHAMD::post_force(int)
     list_step = neighbor->lastcall;
   }
   check(eph_set_timestep(ctx, update->ntimestep));
-  check(eph_post_force(ctx, &atom->x[0][0], &atom->v[0][0], &atom->f[0][0]));
+  if (!multi) {
+    check(eph_post_force(ctx, &atom->x[0][0], &atom->v[0][0], &atom->f[0][0]));
+    return;
+  }
+  // xi, rho, w of the locals, then to the ghosts of every rank, then the rest
+  const int nlocal = atom->nlocal, nghost = atom->nghost;
+  halo.resize(static_cast<size_t>(EPH_HALO_COLS) * std::max(nlocal, 1));
+  halo_ghost.resize(static_cast<size_t>(EPH_HALO_COLS) * std::max(nghost, 1));
+  check(eph_post_force_local(ctx, &atom->x[0][0], &atom->v[0][0], halo.data()));
+  comm->forward_comm(this);
+  check(eph_post_force_ghosts(ctx, halo_ghost.data(), &atom->f[0][0]));
+}
+
+int FixEPHAMD::pack_forward_comm(int n, int *list_, double *data, int, int *)
+{
+  int m = 0;
+  for (int k = 0; k < n; ++k) {
+    const double *row = &halo[static_cast<size_t>(EPH_HALO_COLS) * list_[k]];
+    for (int c = 0; c < EPH_HALO_COLS; ++c) data[m++] = row[c];
+  }
+  return m;
+}
+
+void FixEPHAMD::unpack_forward_comm(int n, int first, double *data)
+{
+  const int nlocal = atom->nlocal;
+  int m = 0;
+  for (int i = first; i < first + n; ++i) {
+    double *row = &halo_ghost[static_cast<size_t>(EPH_HALO_COLS) * (i - nlocal)];
+    for (int c = 0; c < EPH_HALO_COLS; ++c) row[c] = data[m++];
+  }
 }
 
 void FixEPHAMD::final_integrate()
@@ -146,7 +194,16 @@ void FixEPHAMD::final_integrate()
 
 void FixEPHAMD::end_of_step()
 {
-  check(eph_end_of_step(ctx, &atom->x[0][0], &atom->v[0][0], update->ntimestep));
+  if (!multi) {
+    check(eph_end_of_step(ctx, &atom->x[0][0], &atom->v[0][0], update->ntimestep));
+  } else {
+    // this rank's deposits, summed over the ranks, then the same solve on all
+    double E = 0.0;
+    check(eph_end_of_step_local(ctx, &atom->x[0][0], &atom->v[0][0], dT.data(), &E));
+    MPI_Allreduce(MPI_IN_PLACE, dT.data(), static_cast<int>(dT.size()), MPI_DOUBLE, MPI_SUM, world);
+    MPI_Allreduce(MPI_IN_PLACE, &E, 1, MPI_DOUBLE, MPI_SUM, world);
+    check(eph_end_of_step_reduced(ctx, dT.data(), E, update->ntimestep, comm->me == 0));
+  }
   if (atom->nlocal > 0) check(eph_get_peratom(ctx, &array[0][0]));
 }
 
@@ -159,7 +216,7 @@ double FixEPHAMD::compute_vector(int i)
 
 void FixEPHAMD::post_run()
 {
-  check(eph_post_run(ctx));
+  if (comm->me == 0) check(eph_post_run(ctx));  // fix_eph.cpp:1020, rank 0 writes
 }
 
 void FixEPHAMD::grow_arrays(int nmax)
@@ -230,6 +287,7 @@ void FixEPHAMD::upload_atoms()
   const int nlocal = atom->nlocal, nghost = atom->nghost;
   check(eph_set_atoms(ctx, nlocal, nghost, &atom->x[0][0], &atom->v[0][0], atom->type,
                       atom->mask, atom->tag));
+  if (multi) return;  // ghost values arrive through forward_comm instead
   owner.resize(nghost);
   for (int g = 0; g < nghost; ++g) {
     const int o = atom->map(atom->tag[nlocal + g]);

@@ -8,7 +8,11 @@
  * (fix_eph_atomic.cpp:59-269).  The exp-filtered coloured styles and
  * eph/atomic keep per-atom state from step to step (filter states, electronic
  * energies); the adaptor holds it in LAMMPS per-atom rows while atoms are
- * sorted or exchanged, as the reference fixes' grow/copy callbacks do. */
+ * sorted or exchanged, as the reference fixes' grow/copy callbacks do.  Over
+ * several MPI ranks (styles eph, eph/coloured, eph/coloured/exp, model 4) each
+ * rank's context runs its own atoms; the ghosts' xi, rho and w come through
+ * comm->forward_comm and the grid's deposits and the tally through
+ * MPI_Allreduce, as FixEPH and EPH_FDM exchange them. */
 #ifdef FIX_CLASS
 // clang-format off
 FixStyle(eph/amd,FixEPHAMD)
@@ -47,6 +51,8 @@ class FixEPHAMD : public Fix {
   void grow_arrays(int) override;
   void pre_exchange() override;
   void copy_arrays(int, int, int) override;
+  int pack_forward_comm(int, int *, double *, int, int *) override;
+  void unpack_forward_comm(int, int, double *) override;
 
  private:
   eph_context *ctx;
@@ -61,6 +67,10 @@ class FixEPHAMD : public Fix {
   double **state;
   bool state_on_host;  // the rows hold the context's state (since pre_exchange)
   std::vector<double> buf;
+  // several ranks: the halo rows of the locals and the ghosts (EPH_HALO_COLS
+  // each), the grid's deposits being summed, and its size
+  bool multi;
+  std::vector<double> halo, halo_ghost, dT;
   void state_to_host();
   void state_to_device();
 
