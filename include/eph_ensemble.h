@@ -14,7 +14,9 @@
  *     replica's noise stream (seed, tag, step), T_e cell lookup and energy
  *     partial sums are those of a separate run;
  *   - the T_e grids are concatenated and advanced by one stencil launch per
- *     sub-step (replicas with fewer sub-steps copy);
+ *     sub-step (replicas with fewer sub-steps copy); with temperature-
+ *     dependent C_e(T), kappa(T) the counts are found on the device each step
+ *     and the host reads the largest before the stencil launches;
  *   - the whole step can be recorded once as a hipGraph and replayed.
  * Neighbour rebuilds are collective: when any atom of any replica has moved
  * more than skin/2, every replica rebuilds (the steps are logged, so a
@@ -40,7 +42,8 @@ const char* eph_ensemble_last_error(const eph_ensemble* e);
 /* Add a replica: the harness system of cfg and a fix built from the fix
  * command words arg (as eph_create takes them) on that system's box.  All
  * replicas must share the .beta file, flags, model, dt and masses; each has
- * its own seed and T_e grid (constant C_e and kappa).  masses has ntypes + 1
+ * its own seed and T_e grid (C_e and kappa constant or temperature-dependent,
+ * as its grid file says).  masses has ntypes + 1
  * entries in the layout of eph_set_masses (LAMMPS atom->mass, [0] unused). */
 int eph_ensemble_add(eph_ensemble* e, const eph_md_config* cfg, int narg, const char* const* arg,
                      int ntypes, double dt, double boltz, double ftm2v, const double* masses);
@@ -54,15 +57,21 @@ int eph_ensemble_setup(eph_ensemble* e, long long first_step);
  * rebuilt[k] (nsteps entries, may be NULL) is 1 when step first + k rebuilt. */
 int eph_ensemble_run(eph_ensemble* e, int nsteps, int use_graph, double* ms, int* rebuilt);
 
-/* Time the two forms of the density and force sweeps (0: one chunk per
- * 256-thread block, tables in global memory; 1: persistent 1024-thread blocks,
- * tables in LDS) on the current system and keep the faster for later steps.
- * variants[2] = chosen forms (density, force); ms[4] = average times of
- * density form 0, 1, force form 0, 1 (may be NULL).  Both forms give the same
- * bits; the state of the replicas is unchanged. */
+/* Forms of the density and force sweeps, numbered as eph_set_sweep_forms
+ * numbers them: 0 one chunk of a replica per 256-thread block, tables read
+ * through the caches; 1, 2, 3 persistent blocks of 1024, 512, 256 threads with
+ * the tables in LDS, walking tiles of 4, 2, 1 chunks; 4, 5, 6 form 0 held to
+ * 4, 3, 2 blocks per CU.  Every form gives the same bits. */
+#define EPH_ENSEMBLE_FORMS 7
+
+/* Time every form of the two sweeps on the current system and keep the fastest
+ * of each for later steps.  variants[2] = chosen forms (density, force);
+ * ms[2 * EPH_ENSEMBLE_FORMS] = average time of each density form, then each
+ * force form, -1 for a form that cannot run here (may be NULL).  The state of
+ * the replicas is unchanged. */
 int eph_ensemble_autotune(eph_ensemble* e, int* variants, double* ms);
 
-/* pin the forms instead (0 or 1 each) */
+/* pin the forms instead (0 .. EPH_ENSEMBLE_FORMS - 1 each) */
 int eph_ensemble_set_variants(eph_ensemble* e, int density, int force);
 
 /* replicas, total local atoms, total T_e cells */

@@ -126,16 +126,20 @@ def test_ensemble_fdm_substeps_differ_per_replica(tmp_path):
 
 
 def test_ensemble_sweep_forms_give_the_same_bits(tmp_path):
-    """the chunk-per-block and persistent LDS-table forms of the sweeps, and the
-    autotuner's choice, all give the same trajectory"""
+    """every form of the sweeps (chunk per block at any occupancy, persistent
+    LDS-table blocks of 1024, 512 and 256 threads), mixed, and the autotuner's
+    choice, all give the same trajectory"""
     species, dt = 2, 5e-4
     reps = _replicas(str(tmp_path), SPECS, species, dt)
+    names = ea.Ensemble._FORM_NAMES
     runs = []
-    for forms in (("chunk", "chunk"), ("lds", "lds"), ("chunk", "lds"), None):
+    for forms in [(f, f) for f in names] + [("chunk2", "lds256"), None]:
+        print("ensemble forms:", forms, flush=True)
         ens = _ensemble(reps, species, dt)
         if forms is None:
             tuned = ens.autotune()
-            assert tuned["density"] in ("chunk", "lds") and tuned["force"] in ("chunk", "lds")
+            assert set(tuned["density_ms"]) == set(names) == set(tuned["force_ms"]), tuned
+            assert tuned["density"] in names and tuned["force"] in names
         else:
             ens.set_variants(*forms)
         ens.run(15, graph=True)
@@ -145,3 +149,64 @@ def test_ensemble_sweep_forms_give_the_same_bits(tmp_path):
         for r in range(len(reps)):
             for q in range(4):
                 assert np.array_equal(runs[0][r][q], other[r][q]), (r, q)
+
+
+def _tdyn_replicas(tmp, specs, species):
+    """replicas whose grid files make some cells temperature-dependent: C_e(T)
+    and kappa(T) from one parameter file (eph_fdm.h:73-104), a hot spot so the
+    stability extrema, and with them the sub-step counts, change from step to
+    step; a replica with "tdyn": False keeps constant cells"""
+    Tk = np.arange(2000) * 10.0
+    pp = os.path.join(tmp, "tdyn.params")
+    inputs.write_params(pp, 10.0, 3e-6 + 1e-9 * Tk, 0.1 + 2e-4 * Tk)
+    out = []
+    for k, sp in enumerate(specs):
+        n = sp["cells"]
+        beta, names, _ = workload_files(tmp, f"t{k}", species, n, A, n)
+        if k > 0:
+            beta = out[0][1][16]
+        L = n * A
+        idx = np.indices((n, n, n))[::-1].reshape(3, -1).T
+        c = (idx + 0.5) * (L / n) - 0.5 * L
+        T = 300.0 + 1700.0 * np.exp(-(c * c).sum(1) / 32.0)
+        tdyn = (np.arange(n ** 3) % 3 != 0).astype(int) if sp["tdyn"] else 0
+        tin = os.path.join(tmp, f"t{k}.Tin")
+        inputs.write_grid(tin, n, n, n, (0.0, L, 0.0, L, 0.0, L), T, rho=1.0, C_e=3e-6, kappa=0.1,
+                          tdyn=tdyn, params=pp if sp["tdyn"] else "NULL")
+        args = inputs.fix_args(beta, names, seed=sp["fix_seed"], flags="7", grid=(n, n, n),
+                               T_infile=tin)
+        kw = dict(cells=n, a=A, ntypes=species, seed=sp["seed"], T=300.0, pka_eV=sp["pka"])
+        out.append((kw, args))
+    return out
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_ensemble_temperature_dependent_grids_match_separate_runs(tmp_path, graph):
+    """C_e(T), kappa(T) in an ensemble: each replica's sub-step count is found
+    on the device every step (in the single-grid arithmetic) and every replica
+    still follows its separate run bit for bit"""
+    print(f"ensemble: T-dependent grids, graph={graph}", flush=True)
+    species, dt = 1, 5e-4
+    specs = [dict(cells=4, seed=31, fix_seed=301, pka=150.0, tdyn=True),
+             dict(cells=5, seed=32, fix_seed=302, pka=0.0, tdyn=False),
+             dict(cells=6, seed=33, fix_seed=303, pka=200.0, tdyn=True)]
+    reps = _tdyn_replicas(str(tmp_path), specs, species)
+    ens = _ensemble(reps, species, dt)
+    counts = [[ens.info(r)["substeps"] for r in range(3)]]
+    rebuilt = []
+    for n in (1, 6, 13):
+        res = ens.run(n, graph=graph)
+        rebuilt.extend(res["rebuilt"].tolist())
+        counts.append([ens.info(r)["substeps"] for r in range(3)])
+    counts = np.array(counts)
+    assert len(set(counts[:, 0])) > 1 and len(set(counts[:, 2])) > 1, counts  # T-dependent
+    assert len(set(counts[:, 1])) == 1, counts  # constant cells
+    for r, (kw, args) in enumerate(reps):
+        md, fix = _separate(kw, args, species, dt, np.array(rebuilt))
+        n = ens.info(r)["nlocal"]
+        xs, vs, _ = fix.dev_atoms()
+        assert np.array_equal(ens.get(r, "x"), xs[:n]), f"replica {r}: x"
+        assert np.array_equal(ens.get(r, "v"), vs[:n]), f"replica {r}: v"
+        assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), f"replica {r}: T_e"
+        E = ens.get(r, "E")
+        assert E[1] == fix.compute_vector(0), f"replica {r}: Ee"

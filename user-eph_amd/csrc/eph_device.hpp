@@ -290,7 +290,12 @@ hipError_t launch_energy_reduce_batch(const double* partial, const int* chunk0, 
                                       int nrep, double* E_local, double* Ee, hipStream_t s);
 hipError_t launch_fdm_substep_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
                                     const double* Tin, double* Tout, const double* inner_dt,
-                                    const int* steps, int n, int G, hipStream_t s);
+                                    const int* steps, int n, int G, bool tdyn, hipStream_t s);
+hipError_t launch_grid_tdyn_update_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
+                                         int G, hipStream_t s);
+hipError_t launch_fdm_substeps_batch(const GridView* grids, const int8_t* rep_tdyn, const double* grid_dt,
+                                     const int* grid_steps, int* sub_n, double* inner_dt, int* max_sub,
+                                     int R, hipStream_t s);
 hipError_t launch_copy_double(const double* src, double* dst, int n, hipStream_t s);
 hipError_t launch_ghost_xv_dev(const AtomsView& a, const int* owner, const double4* shift,
                                const int* ng, int cap, hipStream_t s);

@@ -59,6 +59,13 @@ struct eph_ensemble {
   DevArray<double> inner_dt, E_local, Ee;
   DevArray<double> T, Tn, dT, S, rho, C, kappa;
   DevArray<int8_t> flag, tdyn;
+  // temperature-dependent C_e, kappa: the sub-step counts are found on the
+  // device each step and the largest one is read back before the stencil runs
+  bool any_tdyn = false;
+  DevArray<int8_t> rep_tdyn;
+  DevArray<double> grid_dt;
+  DevArray<int> grid_steps, max_sub_dev;
+  int* max_sub_host = nullptr;  // pinned
   DepositBuffers dep;
   GridView gall;  // the concatenated grid as one flat array (deposit, clears)
   // recorded step
@@ -79,10 +86,13 @@ struct eph_ensemble {
       eph_destroy(r.fix);
       eph_md_destroy(r.md);
     }
+    if (max_sub_host) (void)hipHostFree(max_sub_host);
     eph_destroy(sys);
   }
   int setup(long long first);
   int enqueue_step(int nlog);
+  int enqueue_fdm(int nsub);
+  int fdm_after_counts();
   int record(int nlog);
   int autotune(double* ms);
 };
@@ -110,7 +120,7 @@ int eph_ensemble::setup(long long first) {
     eph_md* m = p.md;
     p.atom_off = nl; p.nlocal = m->nlocal; nl += m->nlocal;
     eph_grid* g = p.fix->grid;
-    if (g->any_tdyn) return fail(EPH_ERR_UNSUPPORTED, "ensemble replicas need constant C_e and kappa grids");
+    any_tdyn = any_tdyn || g->any_tdyn;
     p.cell_off = G; p.ncells = (int)g->h.ntotal(); G += p.ncells;
     p.chunk0 = nchunk; p.nchunks = (m->nlocal + kBlock - 1) / kBlock; nchunk += p.nchunks;
     g->refresh_minmax();
@@ -216,7 +226,9 @@ int eph_ensemble::setup(long long first) {
   EPH_HIP(this, kappa.reserve(G)); EPH_HIP(this, flag.reserve(G)); EPH_HIP(this, tdyn.reserve(G));
   std::vector<GridView> gv(R);
   std::vector<int> crep(G), coff(R), c0(R), nc(R), subn(R);
-  std::vector<double> idt(R);
+  std::vector<double> idt(R), gdt(R);
+  std::vector<int> gsteps(R);
+  std::vector<int8_t> rtd(R);
   std::vector<uint32_t> sd(R);
   std::vector<Chunk> ch(nchunk);
   for (int r = 0; r < R; ++r) {
@@ -240,6 +252,7 @@ int eph_ensemble::setup(long long first) {
     std::fill(crep.begin() + o, crep.begin() + o + n, r);
     coff[r] = p.cell_off; c0[r] = p.chunk0; nc[r] = p.nchunks;
     subn[r] = (int)p.substeps; idt[r] = p.inner_dt;
+    gdt[r] = g->h.dt; gsteps[r] = g->h.steps; rtd[r] = g->any_tdyn ? 1 : 0;
     sd[r] = static_cast<uint32_t>(p.fix->seed);
     for (int b = 0; b < p.nchunks; ++b) {
       Chunk c;
@@ -255,6 +268,13 @@ int eph_ensemble::setup(long long first) {
   EPH_HIP(this, inner_dt.reserve(R)); EPH_HIP(this, seeds.reserve(R)); EPH_HIP(this, chunks.reserve(nchunk));
   EPH_HIP(this, E_local.reserve(R)); EPH_HIP(this, Ee.reserve(R)); EPH_HIP(this, step_dev.reserve(2));
   EPH_HIP(this, nonfinite.reserve(1));
+  EPH_HIP(this, rep_tdyn.reserve(R)); EPH_HIP(this, grid_dt.reserve(R));
+  EPH_HIP(this, grid_steps.reserve(R)); EPH_HIP(this, max_sub_dev.reserve(1));
+  if (!max_sub_host) EPH_HIP(this, hipHostMalloc(reinterpret_cast<void**>(&max_sub_host), sizeof(int)));
+  *max_sub_host = max_sub;
+  EPH_HIP(this, hipMemcpyAsync(rep_tdyn.p, rtd.data(), R * sizeof(int8_t), hipMemcpyHostToDevice, s));
+  EPH_HIP(this, hipMemcpyAsync(grid_dt.p, gdt.data(), R * sizeof(double), hipMemcpyHostToDevice, s));
+  EPH_HIP(this, hipMemcpyAsync(grid_steps.p, gsteps.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
   EPH_HIP(this, hipMemcpyAsync(gviews.p, gv.data(), R * sizeof(GridView), hipMemcpyHostToDevice, s));
   EPH_HIP(this, hipMemcpyAsync(cell_rep.p, crep.data(), G * sizeof(int), hipMemcpyHostToDevice, s));
   EPH_HIP(this, hipMemcpyAsync(cell_off.p, coff.data(), R * sizeof(int), hipMemcpyHostToDevice, s));
@@ -322,23 +342,53 @@ int eph_ensemble::enqueue_step(int nlog) {
     EPH_HIP(this, launch_fill_double(E_local.p, R, 0.0, s));
   }
   if (sys->flags & EPH_FDM) {
-    for (int n = 0; n < max_sub; ++n) {
-      const double* in = (n % 2 == 0) ? T.p : Tn.p;
-      double* out = (n % 2 == 0) ? Tn.p : T.p;
-      EPH_HIP(this, launch_fdm_substep_batch(gviews.p, cell_rep.p, cell_off.p, in, out, inner_dt.p,
-                                             sub_n.p, n, G, s));
-    }
-    if (max_sub % 2 == 1) EPH_HIP(this, launch_copy_double(Tn.p, T.p, G, s));
-    EPH_HIP(this, launch_fill_double(dT.p, G, 0.0, s));
-    EPH_HIP(this, launch_nonfinite(T.p, G, nonfinite.p, s));
+    if (!any_tdyn) return enqueue_fdm(max_sub);
+    // C_e(T), kappa(T), then each replica's sub-step count (eph_fdm.h:280-314);
+    // the stencil launches follow once the host has read the largest count
+    EPH_HIP(this, launch_grid_tdyn_update_batch(gviews.p, cell_rep.p, cell_off.p, G, s));
+    EPH_HIP(this, launch_fill_int(max_sub_dev.p, 1, 0, s));
+    EPH_HIP(this, launch_fdm_substeps_batch(gviews.p, rep_tdyn.p, grid_dt.p, grid_steps.p, sub_n.p,
+                                            inner_dt.p, max_sub_dev.p, R, s));
+    EPH_HIP(this, hipMemcpyAsync(max_sub_host, max_sub_dev.p, sizeof(int), hipMemcpyDeviceToHost, s));
   }
   return EPH_OK;
 }
 
-// Time each form of the two neighbour sweeps on the current system and keep
-// the faster.  The density sweep only rewrites its own outputs; the force
-// sweep is timed with the force application switched off, so f is untouched.
-// Either way the next step recomputes everything these launches wrote.
+// the stencil sub-steps of every replica: nsub launches, replicas with fewer
+// sub-steps copy in the extra ones; then dT_e is cleared for the next step
+int eph_ensemble::enqueue_fdm(int nsub) {
+  hipStream_t s = sys->stream;
+  for (int n = 0; n < nsub; ++n) {
+    const double* in = (n % 2 == 0) ? T.p : Tn.p;
+    double* out = (n % 2 == 0) ? Tn.p : T.p;
+    EPH_HIP(this, launch_fdm_substep_batch(gviews.p, cell_rep.p, cell_off.p, in, out, inner_dt.p,
+                                           sub_n.p, n, G, any_tdyn, s));
+  }
+  if (nsub % 2 == 1) EPH_HIP(this, launch_copy_double(Tn.p, T.p, G, s));
+  EPH_HIP(this, launch_fill_double(dT.p, G, 0.0, s));
+  EPH_HIP(this, launch_nonfinite(T.p, G, nonfinite.p, s));
+  return EPH_OK;
+}
+
+// temperature-dependent grids: wait for the step's largest sub-step count
+// and run the stencil
+int eph_ensemble::fdm_after_counts() {
+  if (!any_tdyn || !(sys->flags & EPH_FDM)) return EPH_OK;
+  EPH_HIP(this, hipStreamSynchronize(sys->stream));
+  const int n = *max_sub_host;
+  if (n < 1 || n > (1 << 22))
+    return fail(EPH_ERR_NONFINITE, "T_e sub-step count out of range (" + std::to_string(n) +
+                                       "): C_e, rho_e or kappa of a grid is not usable");
+  return enqueue_fdm(n);
+}
+
+// Time every form of the two neighbour sweeps on the current system and keep
+// the fastest of each: one chunk per 256-thread block (tables through the
+// caches), held to 4, 3 or 2 blocks per CU, or persistent blocks of 1024, 512
+// or 256 threads with the tables in LDS.  The density sweep only rewrites its
+// own outputs; the force sweep is timed with the force application switched
+// off, so f is untouched.  Either way the next step recomputes everything
+// these launches wrote.
 int eph_ensemble::autotune(double* ms) {
   hipStream_t s = sys->stream;
   AtomsView a = sys->atoms_view();
@@ -355,32 +405,43 @@ int eph_ensemble::autotune(double* ms) {
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t he = hipEventCreate(&e0);
   if (he == hipSuccess) he = hipEventCreate(&e1);
-  double t[4] = {0, 0, 0, 0};
-  const int nv = batch_lds_possible(sys->tabs) ? 2 : 1;
+  constexpr int nf = EPH_ENSEMBLE_FORMS;
+  double t[2 * nf];
+  std::fill(t, t + 2 * nf, -1.0);  // forms that cannot run here stay at -1
+  const bool lds_ok = batch_lds_possible(sys->tabs);
   for (int phase = 0; phase < 2 && he == hipSuccess; ++phase) {
-    for (int v = 0; v < nv && he == hipSuccess; ++v) {
+    if (phase == 1 && sys->model != 4) break;
+    for (int v = 0; v < nf && he == hipSuccess; ++v) {
+      if (v >= kLdsForm1024 && v <= kLdsForm256 && !lds_ok) continue;
+      double acc = 0.0;
       for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // first launch warms up
         he = hipEventRecord(e0, s);
         if (he == hipSuccess)
           he = phase == 0 ? launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
                                                    step_dev.p, s, v)
-               : sys->model == 4 ? launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p,
-                                                           nchunk, s, v)
-                                 : hipSuccess;
+                          : launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p, nchunk,
+                                                    s, v);
         if (he == hipSuccess) he = hipEventRecord(e1, s);
         if (he == hipSuccess) he = hipEventSynchronize(e1);
         float m = 0;
         if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
-        if (rep > 0) t[2 * phase + v] += m / 3.0;
+        if (rep > 0) acc += m / 3.0;
       }
+      t[phase * nf + v] = acc;
     }
   }
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (he != hipSuccess) return fail_hip(he, "autotune");
-  var_density = (nv > 1 && t[1] < t[0]) ? 1 : 0;
-  var_force = (nv > 1 && t[3] < t[2]) ? 1 : 0;
-  if (ms) std::copy(t, t + 4, ms);
+  auto best = [&](const double* tt) {
+    int b = 0;
+    for (int v = 1; v < nf; ++v)
+      if (tt[v] >= 0.0 && tt[v] < tt[b]) b = v;
+    return b;
+  };
+  var_density = best(t);
+  var_force = sys->model == 4 ? best(t + nf) : 0;
+  if (ms) std::copy(t, t + 2 * nf, ms);
   // a recorded step holds the old forms
   if (exec) { (void)hipGraphExecDestroy(exec); exec = nullptr; }
   if (graph) { (void)hipGraphDestroy(graph); graph = nullptr; }
@@ -523,6 +584,7 @@ int eph_ensemble_run(eph_ensemble* e, int nsteps, int use_graph, double* ms, int
     } else if ((rc = e->enqueue_step(nlog))) {
       return rc;
     }
+    if ((rc = e->fdm_after_counts())) return rc;
   }
   EPH_HIP(e, hipEventRecord(b, s));
   EPH_HIP(e, hipEventSynchronize(b));
@@ -554,7 +616,8 @@ int eph_ensemble_autotune(eph_ensemble* e, int* variants, double* ms) {
 }
 
 int eph_ensemble_set_variants(eph_ensemble* e, int density, int force) {
-  if (!e || density < 0 || density > 1 || force < 0 || force > 1) return EPH_ERR_ARGS;
+  if (!e || density < 0 || density >= EPH_ENSEMBLE_FORMS || force < 0 || force >= EPH_ENSEMBLE_FORMS)
+    return EPH_ERR_ARGS;
   e->var_density = density;
   e->var_force = force;
   if (e->exec) { (void)hipGraphExecDestroy(e->exec); e->exec = nullptr; }
@@ -582,11 +645,20 @@ int eph_ensemble_info(const eph_ensemble* e, int r, int* nlocal, int* ncells, in
   if (nlocal) *nlocal = p.md->nlocal;
   if (ncells) *ncells = (int)p.fix->grid->h.ntotal();
   if (substeps) {
-    double rr = 0, idt = 0;
-    unsigned n = 0;
-    p.fix->grid->refresh_minmax();
-    p.fix->grid->substeps(&rr, &n, &idt);
-    *substeps = (e->reps[0].fix->flags & EPH_FDM) ? (int)n : 0;
+    int n = 0;
+    if (e->ready) {  // the count of the last step (set up on the host, or found on the device)
+      hipStream_t s = e->sys->stream;
+      if (hipMemcpyAsync(&n, e->sub_n.p + r, sizeof(int), hipMemcpyDeviceToHost, s) != hipSuccess ||
+          hipStreamSynchronize(s) != hipSuccess)
+        return EPH_ERR_HIP;
+    } else {
+      double rr = 0, idt = 0;
+      unsigned u = 0;
+      p.fix->grid->refresh_minmax();
+      p.fix->grid->substeps(&rr, &u, &idt);
+      n = (int)u;
+    }
+    *substeps = (e->reps[0].fix->flags & EPH_FDM) ? n : 0;
   }
   return EPH_OK;
 }

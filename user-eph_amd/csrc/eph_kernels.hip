@@ -1692,7 +1692,9 @@ __global__ __launch_bounds__(kBlock) void k_density_w_batch(AtomsView a, TablesV
                                                             const Chunk* __restrict__ chunks,
                                                             const uint32_t* __restrict__ seeds,
                                                             const uint64_t* __restrict__ step) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStageW];
+  // the staging area, sized at launch (a larger request holds fewer blocks per CU)
+  extern __shared__ double2 s_bstage[];
+  double2* s_stage = s_bstage;
   const Chunk c = chunks[blockIdx.x];
   if (static_cast<int>(threadIdx.x & ~63u) >= c.count) return;  // whole wave past the chunk
   StepParams pr = p;
@@ -1720,7 +1722,8 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
                                                              const GridView* __restrict__ grids,
                                                              StepParams p, WorkView w,
                                                              const Chunk* __restrict__ chunks) {
-  __shared__ double2 s_stage[(kBlock / 64) * kCoopStage * kCoopF];
+  extern __shared__ double2 s_bstage[];
+  double2* s_stage = s_bstage;
   const Chunk c = chunks[blockIdx.x];
   if (static_cast<int>(threadIdx.x & ~63u) >= c.count) return;  // whole wave past the chunk
   prl_forces_atom_coop<FRIC, RAND, MULTI>(c.start + threadIdx.x, static_cast<int>(threadIdx.x) < c.count,
@@ -1728,22 +1731,21 @@ __global__ __launch_bounds__(kBlock) void k_prl_forces_batch(AtomsView a, Tables
                                           s_stage + (threadIdx.x >> 6) * kCoopStage * kCoopF);
 }
 
-// Persistent LDS-table forms: each 1024-thread block stages the rho(r^2)
-// tables once and walks tiles of four consecutive chunks (one chunk per 256
-// threads); the tiles of XCD x form one contiguous range.  Same per-atom
-// arithmetic as the chunk-per-block forms.
-constexpr int kChunksPerTile = kBigBlock / kBlock;
-
-template <int XI, bool FRIC, bool MULTI>
-__global__ __launch_bounds__(kBigBlock) void k_density_w_batch_lds(AtomsView a, TablesView t,
+// Persistent LDS-table forms: each block of BS threads (1024, 512 or 256)
+// stages the rho(r^2) tables once and walks tiles of BS / 256 consecutive
+// chunks (one chunk per 256 threads); the tiles of XCD x form one contiguous
+// range.  Same per-atom arithmetic as the chunk-per-block forms.
+template <int XI, bool FRIC, bool MULTI, int BS>
+__global__ __launch_bounds__(BS) void k_density_w_batch_lds(AtomsView a, TablesView t,
                                                                    StepParams p, WorkView w,
                                                                    const Chunk* __restrict__ chunks,
                                                                    int nchunks,
                                                                    const uint32_t* __restrict__ seeds,
                                                                    const uint64_t* __restrict__ step) {
+  constexpr int kChunksPerTile = BS / kBlock;
   extern __shared__ double4 s_tab[];
   const int ntab = t.n_elem * t.n_rho;
-  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  for (int k = threadIdx.x; k < ntab; k += BS) s_tab[k] = t.rho_r2[k];
   __syncthreads();
   const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
   int tile, stp;
@@ -1762,15 +1764,16 @@ __global__ __launch_bounds__(kBigBlock) void k_density_w_batch_lds(AtomsView a, 
   }
 }
 
-template <bool FRIC, bool RAND, bool MULTI>
-__global__ __launch_bounds__(kBigBlock) void k_prl_forces_batch_lds(AtomsView a, TablesView t,
+template <bool FRIC, bool RAND, bool MULTI, int BS>
+__global__ __launch_bounds__(BS) void k_prl_forces_batch_lds(AtomsView a, TablesView t,
                                                                     const GridView* __restrict__ grids,
                                                                     StepParams p, WorkView w,
                                                                     const Chunk* __restrict__ chunks,
                                                                     int nchunks) {
+  constexpr int kChunksPerTile = BS / kBlock;
   extern __shared__ double4 s_tab[];
   const int ntab = t.n_elem * t.n_rho;
-  for (int k = threadIdx.x; k < ntab; k += kBigBlock) s_tab[k] = t.rho_r2[k];
+  for (int k = threadIdx.x; k < ntab; k += BS) s_tab[k] = t.rho_r2[k];
   __syncthreads();
   const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
   int tile, stp;
@@ -1818,6 +1821,7 @@ __global__ __launch_bounds__(1024) void k_energy_reduce_batch(const double* part
 
 // one sub-step over all replicas' cells; a replica whose sub-step count is
 // below n copies its cells, so the two buffers stay in step for everyone
+template <bool TDYN>
 __global__ __launch_bounds__(kBlock) void k_fdm_substep_batch(const GridView* __restrict__ grids,
                                                               const int* __restrict__ cell_rep,
                                                               const int* __restrict__ cell_off,
@@ -1834,7 +1838,79 @@ __global__ __launch_bounds__(kBlock) void k_fdm_substep_batch(const GridView* __
     Tout[gc] = Tin[gc];
     return;
   }
-  fdm_cell<false>(grids[r], Tin + off, Tout + off, gc - off, inner_dt[r]);
+  fdm_cell<TDYN>(grids[r], Tin + off, Tout + off, gc - off, inner_dt[r]);
+}
+
+// C_e(T), kappa(T) of the temperature-dependent cells of every replica
+// (eph_fdm.h:280-287 per replica grid, with that grid's tables)
+__global__ __launch_bounds__(kBlock) void k_grid_tdyn_update_batch(const GridView* __restrict__ grids,
+                                                                   const int* __restrict__ cell_rep,
+                                                                   const int* __restrict__ cell_off,
+                                                                   int G) {
+  const int gc = blockIdx.x * kBlock + threadIdx.x;
+  if (gc >= G) return;
+  const int r = cell_rep[gc];
+  const GridView& g = grids[r];
+  const int c = gc - cell_off[r];
+  if (!g.tdyn[c]) return;
+  const double T = g.T[c];
+  g.C[c] = cubic_at(g.C_T, g.n_CT, g.inv_dT_C, T);
+  g.kappa[c] = cubic_at(g.kappa_T, g.n_kT, g.inv_dT_k, T);
+}
+
+// Sub-step count and inner dt of each replica with temperature-dependent
+// cells, from its stability extrema (smallest C_e and rho_e, largest kappa
+// over non-constant cells seeded with cell 0, eph_fdm.h:289-314), in the
+// arithmetic of the single-grid solve; the other replicas keep the counts set
+// up for them.  Block r handles replica r; *max_sub (cleared before) receives
+// the largest count of the step.
+__global__ __launch_bounds__(256) void k_fdm_substeps_batch(const GridView* __restrict__ grids,
+                                                            const int8_t* __restrict__ rep_tdyn,
+                                                            const double* __restrict__ grid_dt,
+                                                            const int* __restrict__ grid_steps,
+                                                            int* __restrict__ sub_n,
+                                                            double* __restrict__ inner_dt,
+                                                            int* __restrict__ max_sub) {
+  __shared__ double s0[256], s1[256], s2[256];
+  const int r = blockIdx.x;
+  if (rep_tdyn[r]) {
+    const GridView& g = grids[r];
+    const int n = g.nx * g.ny * g.nz;
+    double cmin = g.C[0], rmin = g.rho[0], kmax = g.kappa[0];
+    for (int c = 1 + threadIdx.x; c < n; c += 256) {
+      if (g.flag[c] != 0) {
+        cmin = g.C[c] < cmin ? g.C[c] : cmin;
+        rmin = g.rho[c] < rmin ? g.rho[c] : rmin;
+        kmax = g.kappa[c] > kmax ? g.kappa[c] : kmax;
+      }
+    }
+    s0[threadIdx.x] = cmin; s1[threadIdx.x] = rmin; s2[threadIdx.x] = kmax;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h) {
+        s0[threadIdx.x] = s0[threadIdx.x + h] < s0[threadIdx.x] ? s0[threadIdx.x + h] : s0[threadIdx.x];
+        s1[threadIdx.x] = s1[threadIdx.x + h] < s1[threadIdx.x] ? s1[threadIdx.x + h] : s1[threadIdx.x];
+        s2[threadIdx.x] = s2[threadIdx.x + h] > s2[threadIdx.x] ? s2[threadIdx.x + h] : s2[threadIdx.x];
+      }
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+      const double dt = grid_dt[r];
+      double idt = dt / static_cast<double>(grid_steps[r]);
+      const double dtdxdydz = idt * (1.0 / g.dx / g.dx + 1.0 / g.dy / g.dy + 1.0 / g.dz / g.dz);
+      const double q = dtdxdydz / s0[0] / s1[0] * s2[0];
+      unsigned ns = static_cast<unsigned>(grid_steps[r]);
+      if (q > 0.4) {
+        idt = 0.4 * idt / q;
+        const unsigned k = static_cast<unsigned>(dt / idt);
+        ns = k > 1u ? k : 1u;
+        idt = dt / ns;
+      }
+      sub_n[r] = static_cast<int>(ns);
+      inner_dt[r] = idt;
+    }
+  }
+  if (threadIdx.x == 0) atomicMax(max_sub, sub_n[r]);
 }
 
 __global__ __launch_bounds__(kBlock) void k_copy_double(const double* __restrict__ src,
@@ -2363,6 +2439,11 @@ bool batch_lds_possible(const TablesView& t) {
   return bytes <= static_cast<size_t>(lds_max);
 }
 
+// the form's block size of a persistent LDS-table launch (forms 1-3), else 0
+static int batch_lds_block(int form) {
+  return form == kLdsForm1024 ? 1024 : form == kLdsForm512 ? 512 : form == kLdsForm256 ? 256 : 0;
+}
+
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w_in, const Chunk* chunks, int nchunks,
                                   const uint32_t* seeds, const uint64_t* step, hipStream_t s,
@@ -2371,13 +2452,18 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
   w.rcand2 = 0.0;  // replicas walk their full lists
   if (nchunks == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
-  if (variant == 1 && batch_lds_possible(t)) {
+  const int bs = batch_lds_block(variant);
+  if (bs > 0 && batch_lds_possible(t)) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-    const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
-#define EPH_DWBL(XI, F, M)                                                                      \
-  k_density_w_batch_lds<XI, F, M><<<lds_grid(k_density_w_batch_lds<XI, F, M>, lds, ntiles), kBigBlock, \
-                                    lds, s>>>(a, t, p, w, chunks, nchunks, seeds, step)
-#define EPH_DWBL_M(XI, F) if (multi) EPH_DWBL(XI, F, true); else EPH_DWBL(XI, F, false)
+    const int ntiles = (nchunks + bs / kBlock - 1) / (bs / kBlock);
+#define EPH_DWBL(XI, F, M, BS)                                                                   \
+  k_density_w_batch_lds<XI, F, M, BS><<<lds_grid(k_density_w_batch_lds<XI, F, M, BS>, lds, ntiles, BS), \
+                                        BS, lds, s>>>(a, t, p, w, chunks, nchunks, seeds, step)
+#define EPH_DWBL_B(XI, F, M)                 \
+  if (bs == 1024) EPH_DWBL(XI, F, M, 1024);  \
+  else if (bs == 512) EPH_DWBL(XI, F, M, 512); \
+  else EPH_DWBL(XI, F, M, 256)
+#define EPH_DWBL_M(XI, F) if (multi) { EPH_DWBL_B(XI, F, true); } else { EPH_DWBL_B(XI, F, false); }
 #define EPH_DWBL_F(XI) if (fric) { EPH_DWBL_M(XI, true); } else { EPH_DWBL_M(XI, false); }
     switch (p.xi_mode) {
       case 1: EPH_DWBL_F(1); break;
@@ -2386,10 +2472,12 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
     }
 #undef EPH_DWBL_F
 #undef EPH_DWBL_M
+#undef EPH_DWBL_B
 #undef EPH_DWBL
     return hipGetLastError();
   }
-#define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, 0, s>>>(a, t, p, w, chunks, seeds, step)
+  const size_t dl = chunk_lds(variant, kDensityStageBytes);
+#define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, dl, s>>>(a, t, p, w, chunks, seeds, step)
 #define EPH_DWB_M(XI, F) if (multi) EPH_DWB(XI, F, true); else EPH_DWB(XI, F, false)
 #define EPH_DWB_F(XI) if (fric) { EPH_DWB_M(XI, true); } else { EPH_DWB_M(XI, false); }
   switch (p.xi_mode) {
@@ -2408,21 +2496,28 @@ hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, cons
                                    int nchunks, hipStream_t s, int variant) {
   const bool fric = p.flags & 0x01, rnd = p.flags & 0x02, multi = t.n_elem > 1;
   if (nchunks == 0 || (!fric && !rnd)) return hipSuccess;
-  if (variant == 1 && batch_lds_possible(t)) {
+  const int bs = batch_lds_block(variant);
+  if (bs > 0 && batch_lds_possible(t)) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
-    const int ntiles = (nchunks + kChunksPerTile - 1) / kChunksPerTile;
-#define EPH_PFBL(F, R, M)                                                                        \
-  k_prl_forces_batch_lds<F, R, M><<<lds_grid(k_prl_forces_batch_lds<F, R, M>, lds, ntiles), kBigBlock, \
-                                    lds, s>>>(a, t, grids, p, w, chunks, nchunks)
-#define EPH_PFBL_M(F, R) if (multi) EPH_PFBL(F, R, true); else EPH_PFBL(F, R, false)
+    const int ntiles = (nchunks + bs / kBlock - 1) / (bs / kBlock);
+#define EPH_PFBL(F, R, M, BS)                                                                     \
+  k_prl_forces_batch_lds<F, R, M, BS><<<lds_grid(k_prl_forces_batch_lds<F, R, M, BS>, lds, ntiles, BS), \
+                                        BS, lds, s>>>(a, t, grids, p, w, chunks, nchunks)
+#define EPH_PFBL_B(F, R, M)                  \
+  if (bs == 1024) EPH_PFBL(F, R, M, 1024);   \
+  else if (bs == 512) EPH_PFBL(F, R, M, 512); \
+  else EPH_PFBL(F, R, M, 256)
+#define EPH_PFBL_M(F, R) if (multi) { EPH_PFBL_B(F, R, true); } else { EPH_PFBL_B(F, R, false); }
     if (fric && rnd) { EPH_PFBL_M(true, true); }
     else if (fric) { EPH_PFBL_M(true, false); }
     else { EPH_PFBL_M(false, true); }
 #undef EPH_PFBL_M
+#undef EPH_PFBL_B
 #undef EPH_PFBL
     return hipGetLastError();
   }
-#define EPH_PFB(F, R, M) k_prl_forces_batch<F, R, M><<<nchunks, kBlock, 0, s>>>(a, t, grids, p, w, chunks)
+  const size_t fl = chunk_lds(variant, kForceStageBytes);
+#define EPH_PFB(F, R, M) k_prl_forces_batch<F, R, M><<<nchunks, kBlock, fl, s>>>(a, t, grids, p, w, chunks)
 #define EPH_PFB_M(F, R) if (multi) EPH_PFB(F, R, true); else EPH_PFB(F, R, false)
   if (fric && rnd) { EPH_PFB_M(true, true); }
   else if (fric) { EPH_PFB_M(true, false); }
@@ -2453,9 +2548,29 @@ hipError_t launch_energy_reduce_batch(const double* partial, const int* chunk0, 
 
 hipError_t launch_fdm_substep_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
                                     const double* Tin, double* Tout, const double* inner_dt,
-                                    const int* steps, int n, int G, hipStream_t s) {
+                                    const int* steps, int n, int G, bool tdyn, hipStream_t s) {
   if (G == 0) return hipSuccess;
-  k_fdm_substep_batch<<<nblk(G), kBlock, 0, s>>>(grids, cell_rep, cell_off, Tin, Tout, inner_dt, steps, n, G);
+  if (tdyn)
+    k_fdm_substep_batch<true><<<nblk(G), kBlock, 0, s>>>(grids, cell_rep, cell_off, Tin, Tout, inner_dt,
+                                                         steps, n, G);
+  else
+    k_fdm_substep_batch<false><<<nblk(G), kBlock, 0, s>>>(grids, cell_rep, cell_off, Tin, Tout, inner_dt,
+                                                          steps, n, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_grid_tdyn_update_batch(const GridView* grids, const int* cell_rep, const int* cell_off,
+                                         int G, hipStream_t s) {
+  if (G == 0) return hipSuccess;
+  k_grid_tdyn_update_batch<<<nblk(G), kBlock, 0, s>>>(grids, cell_rep, cell_off, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_fdm_substeps_batch(const GridView* grids, const int8_t* rep_tdyn, const double* grid_dt,
+                                     const int* grid_steps, int* sub_n, double* inner_dt, int* max_sub,
+                                     int R, hipStream_t s) {
+  if (R == 0) return hipSuccess;
+  k_fdm_substeps_batch<<<R, 256, 0, s>>>(grids, rep_tdyn, grid_dt, grid_steps, sub_n, inner_dt, max_sub);
   return hipGetLastError();
 }
 

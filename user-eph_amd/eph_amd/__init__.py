@@ -691,18 +691,24 @@ class Ensemble:
                                            _i(rebuilt) if log else None))
         return dict(ms=ms.value, rebuilt=rebuilt[:nsteps] if log else None)
 
+    # the sweep forms of include/eph_ensemble.h, named as FixEPH names them
+    _FORM_NAMES = FixEPH._FORM_NAMES
+
     def autotune(self):
-        """pick the faster form of the density and force sweeps (same bits either way)"""
+        """time every form of the density and force sweeps and keep the fastest
+        of each (same bits either way); forms that cannot run here are left out"""
+        nf = len(self._FORM_NAMES)
         v = np.zeros(2, np.int32)
-        ms = np.zeros(4)
+        ms = np.zeros(2 * nf)
         self._check(lib().eph_ensemble_autotune(self.h, _i(v), _d(ms)))
-        names = ("chunk", "lds")
+        names = self._FORM_NAMES
         return dict(density=names[v[0]], force=names[v[1]],
-                    density_ms={"chunk": ms[0], "lds": ms[1]}, force_ms={"chunk": ms[2], "lds": ms[3]})
+                    density_ms={names[k]: ms[k] for k in range(nf) if ms[k] >= 0},
+                    force_ms={names[k]: ms[nf + k] for k in range(nf) if ms[nf + k] >= 0})
 
     def set_variants(self, density, force):
-        """pin the sweep forms: 'chunk' or 'lds' each"""
-        f = {"chunk": 0, "lds": 1}
+        """pin the sweep forms by name: chunk, lds, lds512, lds256, chunk4, chunk3, chunk2"""
+        f = {n: k for k, n in enumerate(self._FORM_NAMES)}
         self._check(lib().eph_ensemble_set_variants(self.h, f[density], f[force]))
 
     def size(self):
