@@ -165,6 +165,16 @@ int eph_final_integrate(eph_context* ctx, double* v, const double* f);
 int eph_end_of_step(eph_context* ctx, const double* x, const double* v, long long ntimestep);
 /* FixEPH::post_run, fix_eph.cpp:1019-1021: writes the T_e state file */
 int eph_post_run(eph_context* ctx);
+/* Full-precision binary checkpoint of the fix's own state: the T_e grid, the
+ * energy tally, the last step and the per-atom state of the styles that keep
+ * one (the filter states of eph/coloured/exp*, E_i of eph/atomic), by atom tag.
+ * No counterpart in the reference, whose T.restart (eph_post_run) keeps six
+ * digits of T_e only; a context created with the same command, given the same
+ * atoms and list, that loads this file goes on bit for bit.  Load after
+ * eph_set_atoms (every local tag must be in the file); the file must come from
+ * the same style, model, flags and grid (EPH_ERR_FORMAT otherwise). */
+int eph_checkpoint_save(eph_context* ctx, const char* path);
+int eph_checkpoint_load(eph_context* ctx, const char* path);
 /* FixEPH::compute_vector, fix_eph.cpp:940-948: 0 -> Ee, 1 -> mean T_e
  * (eph/atomic, fix_eph_atomic.cpp:855-864: sum and mean over the group's atoms) */
 int eph_compute_vector(eph_context* ctx, int i, double* out);

@@ -78,6 +78,14 @@ int eph_md_force_rebuild(eph_md* md);
 /* copy the device-resident system (atoms, ghost map, both lists) back into the
  * harness's host arrays, so eph_md_info / eph_md_get describe it */
 int eph_md_pull(eph_md* md, eph_context* ctx);
+/* Full-precision binary checkpoint of a device-resident run: atoms, forces,
+ * ghosts, both lists, the rebuild reference positions and the fix's state
+ * (eph_checkpoint_save).  Load into a harness made with the same
+ * configuration and a fix made with the same command, both attached; the next
+ * eph_md_run then goes on bit for bit.  Errors are reported through
+ * eph_last_error(ctx). */
+int eph_md_checkpoint_save(eph_md* md, eph_context* ctx, const char* path);
+int eph_md_checkpoint_load(eph_md* md, eph_context* ctx, const char* path);
 
 #ifdef __cplusplus
 }

@@ -65,6 +65,8 @@ _SIG = {
     "eph_final_integrate": (C.c_int, [vp, dp, dp]),
     "eph_end_of_step": (C.c_int, [vp, dp, dp, C.c_longlong]),
     "eph_post_run": (C.c_int, [vp]),
+    "eph_checkpoint_save": (C.c_int, [vp, C.c_char_p]),
+    "eph_checkpoint_load": (C.c_int, [vp, C.c_char_p]),
     "eph_compute_vector": (C.c_int, [vp, C.c_int, dp]),
     "eph_get_peratom": (C.c_int, [vp, dp]),
     "eph_peratom_cols": (C.c_int, [vp]),
@@ -126,6 +128,8 @@ _SIG = {
     "eph_md_set_device_rebuild": (C.c_int, [vp, C.c_int]),
     "eph_md_force_rebuild": (C.c_int, [vp]),
     "eph_md_pull": (C.c_int, [vp, vp]),
+    "eph_md_checkpoint_save": (C.c_int, [vp, vp, C.c_char_p]),
+    "eph_md_checkpoint_load": (C.c_int, [vp, vp, C.c_char_p]),
     "eph_ensemble_create": (C.c_int, [C.POINTER(vp), C.c_int]),
     "eph_ensemble_destroy": (None, [vp]),
     "eph_ensemble_last_error": (C.c_char_p, [vp]),
@@ -332,6 +336,13 @@ class FixEPH:
         self._check(lib().eph_end_of_step(self.h, _d(s.x), _d(s.v), ntimestep))
 
     def post_run(self): self._check(lib().eph_post_run(self.h))
+
+    def checkpoint_save(self, path):
+        """full-precision binary checkpoint of the fix's state (T_e, tally, per-atom state)"""
+        self._check(lib().eph_checkpoint_save(self.h, path.encode()))
+
+    def checkpoint_load(self, path):
+        self._check(lib().eph_checkpoint_load(self.h, path.encode()))
 
     def compute_vector(self, i):
         out = C.c_double()
@@ -628,6 +639,16 @@ class MD:
         rc = lib().eph_md_pull(self.h, fix.h)
         if rc != EPH_OK:
             raise EphError(rc, lib().eph_md_last_error(self.h).decode())
+
+    def checkpoint_save(self, fix, path):
+        """full-precision binary checkpoint of the device-resident run (atoms,
+        forces, ghosts, lists and the fix's state)"""
+        fix._check(lib().eph_md_checkpoint_save(self.h, fix.h, path.encode()))
+
+    def checkpoint_load(self, fix, path):
+        fix._check(lib().eph_md_checkpoint_load(self.h, fix.h, path.encode()))
+        inf = self.info()
+        fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
 
     def run(self, fix, nsteps, first_step=1):
         t = np.zeros(4)
