@@ -82,8 +82,10 @@ int eph_md_pull(eph_md* md, eph_context* ctx);
  * ghosts, both lists, the rebuild reference positions and the fix's state
  * (eph_checkpoint_save).  Load into a harness made with the same
  * configuration and a fix made with the same command, both attached; the next
- * eph_md_run then goes on bit for bit.  Errors are reported through
- * eph_last_error(ctx). */
+ * eph_md_run then goes on bit for bit.  path NULL: the harness keeps the
+ * checkpoint in host memory instead (one at a time; load with NULL restores it,
+ * also into the same harness and fix, which replays a segment of a run).
+ * Errors are reported through eph_last_error(ctx). */
 int eph_md_checkpoint_save(eph_md* md, eph_context* ctx, const char* path);
 int eph_md_checkpoint_load(eph_md* md, eph_context* ctx, const char* path);
 

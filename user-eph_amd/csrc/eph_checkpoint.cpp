@@ -37,19 +37,43 @@ namespace {
 
 constexpr char kMagic[8] = {'E', 'P', 'H', 'C', 'K', 'P', 'T', '1'};
 
+// a checkpoint file, or the harness's in-memory checkpoint (mem)
 struct File {
   std::FILE* f = nullptr;
+  std::vector<char>* mem = nullptr;
+  size_t at = 0;
   bool ok = true;
   ~File() {
     if (f) std::fclose(f);
   }
   template <typename T>
   void put(const T* p, size_t n) {
-    if (ok && n && std::fwrite(p, sizeof(T), n, f) != n) ok = false;
+    if (!ok || !n) return;
+    if (mem) {
+      const char* b = reinterpret_cast<const char*>(p);
+      mem->insert(mem->end(), b, b + n * sizeof(T));
+    } else if (std::fwrite(p, sizeof(T), n, f) != n) {
+      ok = false;
+    }
   }
   template <typename T>
   void get(T* p, size_t n) {
-    if (ok && n && std::fread(p, sizeof(T), n, f) != n) ok = false;
+    if (!ok || !n) return;
+    if (mem) {
+      if (at + n * sizeof(T) > mem->size()) { ok = false; return; }
+      std::memcpy(p, mem->data() + at, n * sizeof(T));
+      at += n * sizeof(T);
+    } else if (std::fread(p, sizeof(T), n, f) != n) {
+      ok = false;
+    }
+  }
+  long tell() const { return mem ? static_cast<long>(at) : std::ftell(f); }
+  bool seek(long off, int whence) {
+    if (!mem) return std::fseek(f, off, whence) == 0;
+    const long base = whence == SEEK_SET ? 0 : static_cast<long>(at);
+    if (base + off < 0 || static_cast<size_t>(base + off) > mem->size()) return false;
+    at = static_cast<size_t>(base + off);
+    return true;
   }
 };
 
@@ -220,12 +244,17 @@ int eph_checkpoint_load(eph_context* ctx, const char* path) {
 }
 
 int eph_md_checkpoint_save(eph_md* md, eph_context* fix, const char* path) {
-  if (!md || !fix || !path) return EPH_ERR_ARGS;
+  if (!md || !fix) return EPH_ERR_ARGS;
   if (fix->atomic()) return fix->fail(EPH_ERR_UNSUPPORTED, "harness checkpoints hold the eph styles with a T_e grid");
   if (!fix->have_atoms || !fix->have_list || md->ghost_cap <= 0)
     return fix->fail(EPH_ERR_STATE, "attach the fix to the harness first");
   File fl;
-  if (open_file(fl, path, "wb")) return fix->fail(EPH_ERR_FILE, std::string("cannot write ") + path);
+  if (path) {
+    if (open_file(fl, path, "wb")) return fix->fail(EPH_ERR_FILE, std::string("cannot write ") + path);
+  } else {
+    md->ckpt.clear();
+    fl.mem = &md->ckpt;
+  }
   if (int rc = save_fix(fix, fl, 2)) return rc;
   const int st = fix->stride;
   const int hb[8] = {md->nlocal, fix->nghost, md->ghost_cap, md->cap_f, md->cap_l, st, fix->fstride, 0};
@@ -246,15 +275,20 @@ int eph_md_checkpoint_save(eph_md* md, eph_context* fix, const char* path) {
 }
 
 int eph_md_checkpoint_load(eph_md* md, eph_context* fix, const char* path) {
-  if (!md || !fix || !path) return EPH_ERR_ARGS;
+  if (!md || !fix) return EPH_ERR_ARGS;
   if (fix->atomic()) return fix->fail(EPH_ERR_UNSUPPORTED, "harness checkpoints hold the eph styles with a T_e grid");
   if (!fix->have_atoms || !fix->have_list || md->ghost_cap <= 0)
     return fix->fail(EPH_ERR_STATE, "attach the fix to the harness first");
   File fl;
-  if (open_file(fl, path, "rb")) return fix->fail(EPH_ERR_FILE, std::string("cannot read ") + path);
+  if (path) {
+    if (open_file(fl, path, "rb")) return fix->fail(EPH_ERR_FILE, std::string("cannot read ") + path);
+  } else {
+    if (md->ckpt.empty()) return fix->fail(EPH_ERR_STATE, "no checkpoint kept in memory");
+    fl.mem = &md->ckpt;
+  }
   // the fix's block comes first in the file; its per-atom rows are matched
   // to the atoms restored below, so they are read again after those
-  const long pos0 = std::ftell(fl.f);
+  const long pos0 = fl.tell();
   {
     // skip to the harness block: read the fix block without applying it
     char magic[8];
@@ -265,11 +299,11 @@ int eph_md_checkpoint_load(eph_md* md, eph_context* fix, const char* path) {
     if (!fl.ok || std::memcmp(magic, kMagic, 8) != 0) return fix->fail(EPH_ERR_FORMAT, "not an eph checkpoint file");
     if (hdr[0] != 2) return fix->fail(EPH_ERR_FORMAT, "a fix checkpoint holds no harness system");
     const long G = (long)dims[0] * dims[1] * dims[2];
-    if (G < 0 || std::fseek(fl.f, G * (long)sizeof(double), SEEK_CUR)) return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+    if (G < 0 || !fl.seek(G * (long)sizeof(double), SEEK_CUR)) return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
     fl.get(sz, 2);
     if (!fl.ok || sz[0] < 0 || sz[1] < 0) return fix->fail(EPH_ERR_FORMAT, "checkpoint per-atom block does not match");
     const long skip = (long)sz[0] * (long)sizeof(int) + (long)sz[0] * sz[1] * (long)sizeof(double);
-    if (std::fseek(fl.f, skip, SEEK_CUR)) return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+    if (!fl.seek(skip, SEEK_CUR)) return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
   }
   int hb[8];
   fl.get(hb, 8);
@@ -295,7 +329,7 @@ int eph_md_checkpoint_load(eph_md* md, eph_context* fix, const char* path) {
   fix->drop_candidates();
   fix->pv_fresh = false;
   // now the fix's block, with the restored atoms' tags
-  if (std::fseek(fl.f, pos0, SEEK_SET)) return fix->fail(EPH_ERR_FILE, "cannot reread the checkpoint file");
+  if (!fl.seek(pos0, SEEK_SET)) return fix->fail(EPH_ERR_FILE, "cannot reread the checkpoint file");
   if ((rc = load_fix(fix, fl, 2))) return rc;
   EPH_HIP(fix, hipStreamSynchronize(fix->stream));
   return EPH_OK;

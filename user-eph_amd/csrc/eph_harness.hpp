@@ -41,6 +41,7 @@ struct eph_md {
   eph::DevArray<double> box_d;
   eph::NbBuild nbw;
   int* pinned = nullptr;  // go, nghost, overflow
+  std::vector<char> ckpt;  // eph_md_checkpoint_save(md, ctx, NULL) keeps it here
 
   int fail(int code, const std::string& m) { err = m; return code; }
   int fail_hip(hipError_t e, const char* what) {

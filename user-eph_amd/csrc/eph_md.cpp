@@ -540,8 +540,11 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
                int* nrebuild) {
   if (!md || !ctx || nsteps < 0) return EPH_ERR_ARGS;
   hipStream_t s = ctx->stream;
-  hipEvent_t ev[6];
-  for (auto& e : ev) EPH_HIP(md, hipEventCreate(&e));
+  // five events per step, read after the last step: no step waits for the
+  // previous one to finish, so each step's first kernels are queued behind it
+  std::vector<hipEvent_t> evs(5 * static_cast<size_t>(nsteps));
+  std::vector<char> host_rebuilt(nsteps, 0);
+  for (auto& e : evs) EPH_HIP(md, hipEventCreate(&e));
   double t_fix = 0, t_total = 0, t_pair = 0, t_rebuild = 0;
   int nreb = 0;
   const double lim = 0.25 * md->cfg.skin * md->cfg.skin;
@@ -551,6 +554,7 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
   for (int st = 0; st < nsteps && rc == EPH_OK; ++st) {
     const long long step = first_step + st;
     ctx->step = step;
+    hipEvent_t* ev = evs.data() + 5 * static_cast<size_t>(st);
     EPH_HIP(md, hipEventRecord(ev[0], s));
     if ((rc = ctx->dev_initial_integrate())) break;
     EPH_HIP(md, hipEventRecord(ev[1], s));
@@ -613,20 +617,25 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     // nothing runs between the two hooks here: one pass where the deposit allows
     if ((rc = ctx->dev_final_integrate_end_of_step(step))) break;
     EPH_HIP(md, hipEventRecord(ev[4], s));
-    EPH_HIP(md, hipEventSynchronize(ev[4]));
-    float a = 0, b = 0, c = 0, d = 0;
-    EPH_HIP(md, hipEventElapsedTime(&a, ev[0], ev[1]));
-    EPH_HIP(md, hipEventElapsedTime(&b, ev[3], ev[4]));
-    EPH_HIP(md, hipEventElapsedTime(&c, ev[2], ev[3]));
-    EPH_HIP(md, hipEventElapsedTime(&d, ev[0], ev[4]));
-    float e = 0;
-    if (!host_rebuild) EPH_HIP(md, hipEventElapsedTime(&e, ev[1], ev[2]));
-    t_fix += a + b;
-    t_pair += c;
-    t_total += d;
-    t_rebuild += e;  // device neighbour check / rebuild
+    host_rebuilt[st] = host_rebuild ? 1 : 0;
   }
-  for (auto& e : ev) (void)hipEventDestroy(e);
+  if (rc == EPH_OK && nsteps > 0) {
+    EPH_HIP(md, hipEventSynchronize(evs[5 * static_cast<size_t>(nsteps) - 1]));
+    for (int st = 0; st < nsteps; ++st) {
+      const hipEvent_t* ev = evs.data() + 5 * static_cast<size_t>(st);
+      float a = 0, b = 0, c = 0, d = 0, e = 0;
+      EPH_HIP(md, hipEventElapsedTime(&a, ev[0], ev[1]));
+      EPH_HIP(md, hipEventElapsedTime(&b, ev[3], ev[4]));
+      EPH_HIP(md, hipEventElapsedTime(&c, ev[2], ev[3]));
+      EPH_HIP(md, hipEventElapsedTime(&d, ev[0], ev[4]));
+      if (!host_rebuilt[st]) EPH_HIP(md, hipEventElapsedTime(&e, ev[1], ev[2]));
+      t_fix += a + b;
+      t_pair += c;
+      t_total += d;
+      t_rebuild += e;  // device neighbour check / rebuild
+    }
+  }
+  for (auto& e : evs) (void)hipEventDestroy(e);
   if (rc != EPH_OK) {
     if (md->err.empty()) md->err = ctx->err;
     return rc;

@@ -640,13 +640,14 @@ class MD:
         if rc != EPH_OK:
             raise EphError(rc, lib().eph_md_last_error(self.h).decode())
 
-    def checkpoint_save(self, fix, path):
+    def checkpoint_save(self, fix, path=None):
         """full-precision binary checkpoint of the device-resident run (atoms,
-        forces, ghosts, lists and the fix's state)"""
-        fix._check(lib().eph_md_checkpoint_save(self.h, fix.h, path.encode()))
+        forces, ghosts, lists and the fix's state); path None keeps it in the
+        harness's host memory"""
+        fix._check(lib().eph_md_checkpoint_save(self.h, fix.h, None if path is None else path.encode()))
 
-    def checkpoint_load(self, fix, path):
-        fix._check(lib().eph_md_checkpoint_load(self.h, fix.h, path.encode()))
+    def checkpoint_load(self, fix, path=None):
+        fix._check(lib().eph_md_checkpoint_load(self.h, fix.h, None if path is None else path.encode()))
         inf = self.info()
         fix.nlocal, fix.nghost = inf["nlocal"], inf["nghost"]
 
