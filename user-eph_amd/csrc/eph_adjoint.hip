@@ -13,12 +13,12 @@ namespace eph {
 
 namespace {
 
-// value and d/dx of a table cubic at x (same clamped interval as cubic_at)
-__device__ __forceinline__ void cubic_vd(const double4* __restrict__ tab, int n, double inv,
-                                         double x, double* v, double* dv) {
-  int i = static_cast<int>(x * inv);
-  i = i < 0 ? 0 : (i < n - 1 ? i : n - 1);
-  const double4 c = tab[i];
+// value and d/dr^2 of rho(r^2) of element e at x = r^2 (the interval and cubic
+// of rho_r2_at, interleaved tables)
+__device__ __forceinline__ void rho_r2_vd(const TablesView& t, int e, double x, double* v, double* dv) {
+  int i = static_cast<int>(x * t.inv_dr2);
+  i = i < 0 ? 0 : (i < t.n_rho - 1 ? i : t.n_rho - 1);
+  const double4 c = t.rho_r2[i * t.n_elem + e];
   *v = c.x + x * (c.y + x * (c.z + x * c.w));
   *dv = c.y + x * (2.0 * c.z + 3.0 * x * c.w);
 }
@@ -225,8 +225,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_force(AdjStepView s, AtomsView a
     const double r2 = ex * ex + ey * ey + ez * ez;
     const int ej = multi ? static_cast<int>(pj.w) : 0;
     double vji, dvji, vij, dvij;
-    cubic_vd(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2, &vji, &dvji);
-    cubic_vd(t.rho_r2 + ei * t.n_rho, t.n_rho, t.inv_dr2, r2, &vij, &dvij);
+    rho_r2_vd(t, ej, r2, &vji, &dvji);
+    rho_r2_vd(t, ei, r2, &vij, &dvij);
     const double B1 = alpha_i * vji / (rho_i * r2), B2 = alpha_j * vij / (rho_j * r2);
     double B1b = 0, B2b = 0, gx = 0, gy = 0, gz = 0;  // gx..: d/de
     if (fric) {
@@ -297,7 +297,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_w(AdjStepView s, AtomsView a, Ta
     const double r2 = ex * ex + ey * ey + ez * ez;
     const int ej = multi ? static_cast<int>(pj.w) : 0;
     double vr, dvr;
-    cubic_vd(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2, &vr, &dvr);
+    rho_r2_vd(t, ej, r2, &vr, &dvr);
     const double B = alpha_i * vr / (rho_i * r2);
     const double c1 = ex * vi.x + ey * vi.y + ez * vi.z;
     const double c2 = ex * vj.x + ey * vj.y + ez * vj.z;
@@ -394,7 +394,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_rho(AdjStepView s, AtomsView a, 
     const double r2 = ex * ex + ey * ey + ez * ez;
     const int ej = multi ? static_cast<int>(pj.w) : 0;
     double vr, dvr;
-    cubic_vd(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2, &vr, &dvr);
+    rho_r2_vd(t, ej, r2, &vr, &dvr);
     const double r2b = rb * dvr;
     const double gx = 2.0 * r2b * ex, gy = 2.0 * r2b * ey, gz = 2.0 * r2b * ez;
     xbx -= gx; xby -= gy; xbz -= gz;

@@ -82,7 +82,7 @@ __global__ __launch_bounds__(kBlock) void k_atomic_env(AtomsView a, TablesView t
       const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
       if (r2 < t.rc2) {
-        rho += cubic_at(t.rho_r2 + static_cast<int>(pj.w) * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        rho += rho_r2_at(t.rho_r2, t, static_cast<int>(pj.w), r2);
         w.act[na * stride + i] = j;
         ++na;
       }
@@ -110,7 +110,7 @@ __global__ __launch_bounds__(kBlock) void k_atomic_env(AtomsView a, TablesView t
       const PosVel q = w.pv[j];
       const double ex = q.p.x - pi.x, ey = q.p.y - pi.y, ez = q.p.z - pi.z;
       const double r2 = ex * ex + ey * ey + ez * ez;
-      const double vr = cubic_at(t.rho_r2 + static_cast<int>(q.p.w) * t.n_rho, t.n_rho, t.inv_dr2, r2);
+      const double vr = rho_r2_at(t.rho_r2, t, static_cast<int>(q.p.w), r2);
       const double pre = alpha * vr / (rho * r2);
       const double var1 = pre * (ex * vi.x + ey * vi.y + ez * vi.z);
       const double var2 = pre * (ex * q.v.x + ey * q.v.y + ez * q.v.z);
@@ -183,9 +183,9 @@ __global__ __launch_bounds__(kBlock) void k_atomic_forces(AtomsView a, TablesVie
         const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
         const double r2 = ex * ex + ey * ey + ez * ez;
         const int ej = static_cast<int>(pj.w);
-        const double vr_ji = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        const double vr_ji = rho_r2_at(t.rho_r2, t, ej, r2);
         const double var1 = alpha_i * vr_ji * (ex * rwi.x + ey * rwi.y + ez * rwi.z) / (rho_i * r2);
-        const double vr_ij = cubic_at(t.rho_r2 + ei * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        const double vr_ij = rho_r2_at(t.rho_r2, t, ei, r2);
         const double var2 = alpha_j * vr_ij * (ex * rwj.x + ey * rwj.y + ez * rwj.z) / (rho_j * r2);
         const double dvar = var1 - var2;
         const double f0 = dvar * ex, f1 = dvar * ey, f2 = dvar * ez;
@@ -213,10 +213,10 @@ __global__ __launch_bounds__(kBlock) void k_atomic_forces(AtomsView a, TablesVie
         const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
         const double r2 = ex * ex + ey * ey + ez * ez;
         const int ej = static_cast<int>(pj.w);
-        const double vr_ji = cubic_at(t.rho_r2 + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        const double vr_ji = rho_r2_at(t.rho_r2, t, ej, r2);
         const double var1 =
             v_Ti * alpha_i * vr_ji * (ex * xai.x + ey * xai.y + ez * xai.z) / (rho_i * r2);
-        const double vr_ij = cubic_at(t.rho_r2 + ei * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        const double vr_ij = rho_r2_at(t.rho_r2, t, ei, r2);
         const double var2 =
             v_Tj * alpha_j * vr_ij * (ex * xaj.x + ey * xaj.y + ez * xaj.z) / (rho_j * r2);
         const double dvar = p.eta * (var1 - var2);

@@ -830,7 +830,9 @@ int eph_create(eph_context** out, int narg, const char* const* arg, int ntypes,
   std::vector<Cubic> r2((size_t)ne * nr), al((size_t)ne * nb), be((size_t)ne * nb), r1((size_t)ne * nr);
   for (int e = 0; e < ne; ++e) {
     std::copy(c->beta.rho[e].knots.begin(), c->beta.rho[e].knots.end(), r1.begin() + (size_t)e * nr);
-    std::copy(c->beta.rho_r_sq[e].knots.begin(), c->beta.rho_r_sq[e].knots.end(), r2.begin() + (size_t)e * nr);
+    // rho(r^2) knots interleaved by element: the elements' intervals at one r^2
+    // share a cache line (knot k of element e at k * ne + e)
+    for (int k = 0; k < nr; ++k) r2[(size_t)k * ne + e] = c->beta.rho_r_sq[e].knots[k];
     std::copy(c->beta.alpha[e].knots.begin(), c->beta.alpha[e].knots.end(), al.begin() + (size_t)e * nb);
     std::copy(c->beta.beta[e].knots.begin(), c->beta.beta[e].knots.end(), be.begin() + (size_t)e * nb);
   }

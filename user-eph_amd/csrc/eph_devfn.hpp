@@ -24,6 +24,17 @@ static __device__ __forceinline__ double cubic_at(const double4* __restrict__ ta
   return c.x + x * (c.y + x * (c.z + x * c.w));
 }
 
+// rho(r^2) of element e (EPH_Beta::get_rho_r_sq, eph_beta.h:164-170) from the
+// interleaved tables of TablesView::rho_r2 (or their copy in LDS): the cubic of
+// cubic_at, with the element's knot k at k * n_elem + e
+static __device__ __forceinline__ double rho_r2_at(const double4* __restrict__ tab, const TablesView& t,
+                                                   int e, double r2) {
+  int i = static_cast<int>(r2 * t.inv_dr2);
+  i = i < 0 ? 0 : (i < t.n_rho - 1 ? i : t.n_rho - 1);
+  const double4 c = tab[i * t.n_elem + e];
+  return c.x + r2 * (c.y + r2 * (c.z + r2 * c.w));
+}
+
 // EPH_FDM::get_index (eph_fdm.h:494-509): floor, then periodic fold
 static __device__ __forceinline__ int cell_of(const GridView& g, double x, double y, double z) {
   int lx = static_cast<int>(floor((x - g.x0) / g.dx));

@@ -24,7 +24,7 @@ struct TablesView {
   int n_elem = 0;
   int n_rho = 0;
   double inv_dr2 = 0;         // 1 / d(r^2)
-  const double4* rho_r2 = nullptr;  // [n_elem][n_rho]
+  const double4* rho_r2 = nullptr;  // [n_rho][n_elem]: knot k of element e at k * n_elem + e
   int n_beta = 0;
   double inv_drho = 0;
   const double4* alpha = nullptr;   // [n_elem][n_beta]

@@ -167,7 +167,7 @@ __device__ __forceinline__ void density_w_pair(const double4& pi, const double4&
   const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
   const double r2 = ex * ex + ey * ey + ez * ez;
   const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-  const double vr = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+  const double vr = rho_r2_at(rtab, t, ej, r2);
   const double pre = alpha * vr / (rho * r2);
   const double ev1 = ex * vi.x + ey * vi.y + ez * vi.z;
   const double var1 = pre * ev1;
@@ -240,7 +240,7 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
       const double r2 = ex * ex + ey * ey + ez * ez;
       if (r2 < t.rc2) {
         const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-        rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+        rho += rho_r2_at(rtab, t, ej, r2);
         w.act[na * stride + i] = j;
         ++na;
       }
@@ -371,7 +371,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
           const double r2 = ex * ex + ey * ey + ez * ez;
           if (r2 < t.rc2) {
             const int ej = MULTI ? static_cast<int>(pq[e].w) : 0;
-            rho += cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
+            rho += rho_r2_at(rtab, t, ej, r2);
             w.act[na * stride + i] = jc[e];
             ++na;
           }
@@ -583,8 +583,8 @@ __device__ __forceinline__ void prl_pair(const double4& pi, const double4& rwi, 
   const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
   const double r2 = ex * ex + ey * ey + ez * ez;
   const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-  const double vr_ji = cubic_at(rtab + ej * t.n_rho, t.n_rho, t.inv_dr2, r2);
-  const double vr_ij = MULTI ? cubic_at(rtab + ei * t.n_rho, t.n_rho, t.inv_dr2, r2) : vr_ji;
+  const double vr_ji = rho_r2_at(rtab, t, ej, r2);
+  const double vr_ij = MULTI ? rho_r2_at(rtab, t, ei, r2) : vr_ji;
   if (FRIC) {
     const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
     const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
