@@ -79,10 +79,11 @@ int eph_md_force_rebuild(eph_md* md);
  * harness's host arrays, so eph_md_info / eph_md_get describe it */
 int eph_md_pull(eph_md* md, eph_context* ctx);
 /* Full-precision binary checkpoint of a device-resident run: atoms, forces,
- * ghosts, both lists, the rebuild reference positions and the fix's state
- * (eph_checkpoint_save).  Load into a harness made with the same
- * configuration and a fix made with the same command, both attached; the next
- * eph_md_run then goes on bit for bit.  path NULL: the harness keeps the
+ * ghosts, both lists, the rebuild reference positions, the density sweep's
+ * candidate lists and the fix's state (eph_checkpoint_save).  Load into a
+ * harness made with the same configuration and a fix made with the same
+ * command, both attached; the next eph_md_run then goes on bit for bit, down
+ * to which waves walk their candidates.  path NULL: the harness keeps the
  * checkpoint in host memory instead (one at a time; load with NULL restores it,
  * also into the same harness and fix, which replays a segment of a run).
  * Errors are reported through eph_last_error(ctx). */

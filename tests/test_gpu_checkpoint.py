@@ -146,16 +146,28 @@ def test_harness_checkpoint_resumes_bit_for_bit(tmp_path):
         md.attach(fix)
         return md, fix
 
+    def steps(md, fix, first, n):
+        # one step per call, with the density sweep's candidate-list path of each
+        rebuilds, paths = 0, []
+        for s in range(first, first + n):
+            rebuilds += md.run(fix, 1, first_step=s)["rebuilds"]
+            paths.append(fix.candidate_stats())
+        return rebuilds, paths
+
     mdA, fA = fresh()
-    ra = mdA.run(fA, 30, first_step=1)
+    ra = mdA.run(fA, 15, first_step=1)["rebuilds"]
+    rb, pathsA = steps(mdA, fA, 16, 15)
     mdB, fB = fresh()
-    r1 = mdB.run(fB, 15, first_step=1)
+    r1 = mdB.run(fB, 15, first_step=1)["rebuilds"]
     mdB.checkpoint_save(fB, ck)
     mdC, fC = fresh()
     mdC.checkpoint_load(fC, ck)
-    r2 = mdC.run(fC, 15, first_step=16)
-    assert r1["rebuilds"] >= 1 and r2["rebuilds"] >= 1, (r1, r2)
-    assert r1["rebuilds"] + r2["rebuilds"] == ra["rebuilds"]
+    r2, pathsC = steps(mdC, fC, 16, 15)
+    assert r1 >= 1 and r2 >= 1, (r1, r2)
+    assert r1 == ra and r2 == rb
+    # the candidate lists travel with the checkpoint: the resumed run walks
+    # them where the run that went on did (no full-list retake at the load)
+    assert pathsC == pathsA, (pathsA, pathsC)
     for a, b in zip(fA.dev_atoms(), fC.dev_atoms()):
         assert np.array_equal(a, b)
     assert np.array_equal(fA.grid().get("T"), fC.grid().get("T"))

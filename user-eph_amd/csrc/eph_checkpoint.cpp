@@ -19,6 +19,12 @@
 //   tag [nall_cap]; float64x4 shift[ghost_cap]; int32 owner[ghost_cap];
 //   float64x4 xref[nlocal]; float64 f[3 fstride]; int32 nbr[cap_f stride],
 //   nnbr[stride], lj[cap_l stride], ljn[stride]  (nall_cap = nlocal + ghost_cap)
+//   and, when the last int32 of that header is 1, the density sweep's
+//   candidate lists: int64 sizes n_cand, n_ncand, n_cref; int32 state[8],
+//   ncand[n_ncand], cand[n_cand]; float64x4 cref[n_cref].  They are what the
+//   lists were when the checkpoint was taken, so a restored run takes the
+//   same kernel paths (candidate walk or full list) as the run that went on;
+//   without them the first density sweep after a load takes the full list.
 #include <cstdio>
 #include <cstring>
 #include <memory>
@@ -257,7 +263,9 @@ int eph_md_checkpoint_save(eph_md* md, eph_context* fix, const char* path) {
   }
   if (int rc = save_fix(fix, fl, 2)) return rc;
   const int st = fix->stride;
-  const int hb[8] = {md->nlocal, fix->nghost, md->ghost_cap, md->cap_f, md->cap_l, st, fix->fstride, 0};
+  const bool cands = fix->cstate.p && fix->cand.p && fix->ncand.p && fix->cref.p;
+  const int hb[8] = {md->nlocal, fix->nghost, md->ghost_cap, md->cap_f, md->cap_l, st, fix->fstride,
+                     cands ? 1 : 0};
   fl.put(hb, 8);
   const size_t na = (size_t)md->nlocal + md->ghost_cap, gc = md->ghost_cap;
   int rc;
@@ -271,6 +279,13 @@ int eph_md_checkpoint_save(eph_md* md, eph_context* fix, const char* path) {
       (rc = put_dev(fix, fl, md->lj_nbr.p, (size_t)md->cap_l * st)) ||
       (rc = put_dev(fix, fl, md->lj_nn.p, st)))
     return rc;
+  if (cands) {
+    const long long sz[3] = {(long long)fix->cand.n, (long long)fix->ncand.n, (long long)fix->cref.n};
+    fl.put(sz, 3);
+    if ((rc = put_dev(fix, fl, fix->cstate.p, 8)) || (rc = put_dev(fix, fl, fix->ncand.p, fix->ncand.n)) ||
+        (rc = put_dev(fix, fl, fix->cand.p, fix->cand.n)) || (rc = put_dev(fix, fl, fix->cref.p, fix->cref.n)))
+      return rc;
+  }
   return fl.ok ? EPH_OK : fix->fail(EPH_ERR_FILE, "cannot write the checkpoint file");
 }
 
@@ -323,10 +338,28 @@ int eph_md_checkpoint_load(eph_md* md, eph_context* fix, const char* path) {
       (rc = get_dev(fix, fl, md->lj_nbr.p, (size_t)md->cap_l * st)) ||
       (rc = get_dev(fix, fl, md->lj_nn.p, st)))
     return rc;
+  bool cands = false;
+  if (hb[7] == 1) {
+    long long sz[3];
+    fl.get(sz, 3);
+    if (!fl.ok || sz[0] < 0 || sz[1] < 0 || sz[2] < 0) return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+    cands = fix->cstate.p && fix->cand.p && (size_t)sz[0] == fix->cand.n && (size_t)sz[1] == fix->ncand.n &&
+            (size_t)sz[2] == fix->cref.n;
+    if (cands) {
+      if ((rc = get_dev(fix, fl, fix->cstate.p, 8)) || (rc = get_dev(fix, fl, fix->ncand.p, fix->ncand.n)) ||
+          (rc = get_dev(fix, fl, fix->cand.p, fix->cand.n)) || (rc = get_dev(fix, fl, fix->cref.p, fix->cref.n)))
+        return rc;
+    } else if (!fl.seek((long)(8 + sz[0] + sz[1]) * (long)sizeof(int) + (long)sz[2] * (long)sizeof(double4),
+                        SEEK_CUR)) {
+      return fix->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+    }
+  } else if (hb[7] != 0) {
+    return fix->fail(EPH_ERR_FORMAT, "checkpoint of another harness system (atoms or capacities differ)");
+  }
   fix->nghost = hb[1];
   md->nghost = hb[1];
   EPH_HIP(fix, launch_fill_int(md->flags.p, 8, 0, fix->stream));
-  fix->drop_candidates();
+  if (!cands) fix->drop_candidates();  // other capacities: retaken by the next density sweep
   fix->pv_fresh = false;
   // now the fix's block, with the restored atoms' tags
   if (!fl.seek(pos0, SEEK_SET)) return fix->fail(EPH_ERR_FILE, "cannot reread the checkpoint file");
