@@ -78,6 +78,34 @@ def counter_traffic(natoms):
     return out
 
 
+SWEEP_COUNTER_FILE = os.path.join(ROOT, "profiles", "fix_eph_sweep_counters_fcc80_cascade.txt")
+# MI355X: 256 CUs x 4 SIMDs; a wave64 VALU instruction occupies its SIMD 4
+# cycles (16 lanes per cycle, the FP64 rate); 2.4 GHz peak engine clock
+SIMDS, VALU_CYCLES, CLOCK_HZ = 1024, 4, 2.4e9
+
+
+def counter_valu(natoms):
+    """VALU instructions per wave and the VALU-active share of SIMD cycles of
+    each sweep of the 2,048,000-atom cascade, from the committed SQ counter
+    summary (rocprofv3 --pmc SQ_INSTS_VALU / SQ_WAVES, SQ_ACTIVE_INST_VALU);
+    {} when it is missing or describes another system size."""
+    out = {}
+    if natoms != 2048000 or not os.path.exists(SWEEP_COUNTER_FILE):
+        return out
+    with open(SWEEP_COUNTER_FILE) as f:
+        for line in f:
+            if line.startswith("#") or ":" not in line or "VALU instructions" not in line:
+                continue
+            name, rest = line.split(":", 1)
+            try:
+                per_wave = float(rest.split(" VALU instructions")[0].rsplit(",", 1)[1])
+                active = float(rest.split("VALU active ")[1].split("%")[0]) / 100.0
+            except (IndexError, ValueError):
+                continue
+            out[name.split()[0]] = {"valu_per_wave": per_wave, "valu_active": active}
+    return out
+
+
 def host_cpu_info():
     """CPU model (lscpu / /proc/cpuinfo) and online core count of this host."""
     import subprocess
@@ -486,8 +514,18 @@ def main():
     }
     force_prl["frac"] = force_prl["achieved_GBs"] / HBM_PEAK_GBS
     traffic = counter_traffic(N)
+    valu = counter_valu(N)
     for k, v in sweeps.items():
         v["traffic_bytes_per_launch"] = traffic.get(k)
+        c = valu.get(k)
+        if c:
+            # the launch's VALU instructions issued back to back on every SIMD:
+            # the time the arithmetic alone needs, beside the measured launch
+            floor_ms = ((N + 63) // 64) * c["valu_per_wave"] * VALU_CYCLES / SIMDS / CLOCK_HZ * 1e3
+            v["valu"] = {"instructions_per_wave": c["valu_per_wave"],
+                         "floor_ms": floor_ms, "floor_frac": floor_ms / v["avg_launch_ms"],
+                         "active_frac_counter": c["valu_active"],
+                         "source": os.path.relpath(SWEEP_COUNTER_FILE, ROOT)}
     dom = max(sweeps, key=lambda k: sweeps[k]["avg_launch_ms"])
     achieved = sweeps[dom]["achieved_GBs"]
     kernels = {k: {"avg_ms": v[0] / max(v[1], 1), "launches": v[1]} for k, v in rep.items()}
@@ -536,6 +574,7 @@ def main():
             "fp64": {"achieved": sweeps[dom]["fp64_TFLOPs"], "peak": FP64_PEAK_TFLOPS,
                      "unit": "TFLOP/s", "frac": sweeps[dom]["fp64_frac"],
                      "flops_per_atom": sweeps[dom]["flops_per_atom"]},
+            "valu": sweeps[dom].get("valu"),
             "sweeps": sweeps,
         },
         "cpu_baseline": None,

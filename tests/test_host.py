@@ -278,3 +278,20 @@ def test_lammps_adaptor_compiles_against_the_lammps_api():
                             "-I" + mpi_inc, src],
                            capture_output=True, text=True)
         assert r.returncode == 0, r.stderr
+
+
+def test_bench_reads_the_committed_counter_summaries():
+    """bench.py's roofline line takes the HBM traffic and the VALU instruction
+    counts of the sweeps from the rocprofv3 summaries under profiles/: both
+    parse, for both sweeps, with plausible magnitudes"""
+    import importlib.util
+    spec = importlib.util.spec_from_file_location("bench_mod", os.path.join(ROOT, "bench.py"))
+    bench = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(bench)
+    traffic = bench.counter_traffic(2048000)
+    valu = bench.counter_valu(2048000)
+    for k in ("k_density_w", "k_prl_forces"):
+        assert 1e8 < traffic[k] < 1e10, (k, traffic.get(k))
+        assert 1e3 < valu[k]["valu_per_wave"] < 1e5, (k, valu.get(k))
+        assert 0.0 < valu[k]["valu_active"] <= 1.0
+    assert bench.counter_traffic(4000) == {} and bench.counter_valu(4000) == {}
