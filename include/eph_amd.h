@@ -172,7 +172,9 @@ int eph_post_run(eph_context* ctx);
  * digits of T_e only; a context created with the same command, given the same
  * atoms and list, that loads this file goes on bit for bit.  Load after
  * eph_set_atoms (every local tag must be in the file); the file must come from
- * the same style, model, flags and grid (EPH_ERR_FORMAT otherwise). */
+ * the same style, model, flags and grid (EPH_ERR_FORMAT otherwise).  The whole
+ * file is read and checked before the context changes: a refused file leaves
+ * it as it was. */
 int eph_checkpoint_save(eph_context* ctx, const char* path);
 int eph_checkpoint_load(eph_context* ctx, const char* path);
 /* FixEPH::compute_vector, fix_eph.cpp:940-948: 0 -> Ee, 1 -> mean T_e

@@ -86,6 +86,8 @@ int eph_md_pull(eph_md* md, eph_context* ctx);
  * to which waves walk their candidates.  path NULL: the harness keeps the
  * checkpoint in host memory instead (one at a time; load with NULL restores it,
  * also into the same harness and fix, which replays a segment of a run).
+ * Files of another system or another fix are refused before anything changes;
+ * a file damaged past its headers can leave the harness partly restored.
  * Errors are reported through eph_last_error(ctx). */
 int eph_md_checkpoint_save(eph_md* md, eph_context* ctx, const char* path);
 int eph_md_checkpoint_load(eph_md* md, eph_context* ctx, const char* path);

@@ -123,6 +123,24 @@ def test_checkpoint_refuses_other_files(tmp_path):
     junk.write_text("T.restart is text\n")
     with pytest.raises(ea.EphError, match="not an eph checkpoint"):
         a.checkpoint_load(str(junk))
+    # a per-atom count beyond the file's end, and a file cut short inside the
+    # per-atom block: refused before anything is allocated or changed
+    raw = open(ck, "rb").read()
+    dims = np.frombuffer(raw[56:72], np.int32)
+    off = 72 + 8 * int(dims[0]) * int(dims[1]) * int(dims[2])
+    assert int(np.frombuffer(raw[off:off + 4], np.int32)[0]) == s0.nlocal
+    big = bytearray(raw)
+    big[off:off + 4] = np.int32(1 << 30).tobytes()
+    (tmp_path / "big").write_bytes(bytes(big))
+    (tmp_path / "cut").write_bytes(raw[:off + 8 + 4 * (s0.nlocal // 2)])
+    a.grid().set("T", a.grid().get("T") + 1.0)
+    T_before = a.grid().get("T")
+    for bad in ("big", "cut"):
+        with pytest.raises(ea.EphError, match="ends early"):
+            a.checkpoint_load(str(tmp_path / bad))
+        assert np.array_equal(a.grid().get("T"), T_before), bad
+    a.checkpoint_load(ck)
+    assert np.array_equal(a.grid().get("T") + 1.0, T_before)
 
 
 def test_harness_checkpoint_resumes_bit_for_bit(tmp_path):

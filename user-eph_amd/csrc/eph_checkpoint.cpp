@@ -74,6 +74,16 @@ struct File {
     }
   }
   long tell() const { return mem ? static_cast<long>(at) : std::ftell(f); }
+  // bytes after the current position (-1 if unknown): sizes read from a file
+  // are checked against it before anything is allocated for them
+  long remaining() {
+    if (mem) return static_cast<long>(mem->size() - at);
+    const long here = std::ftell(f);
+    if (here < 0 || std::fseek(f, 0, SEEK_END) != 0) return -1;
+    const long end = std::ftell(f);
+    if (std::fseek(f, here, SEEK_SET) != 0 || end < here) return -1;
+    return end - here;
+  }
   bool seek(long off, int whence) {
     if (!mem) return std::fseek(f, off, whence) == 0;
     const long base = whence == SEEK_SET ? 0 : static_cast<long>(at);
@@ -151,7 +161,8 @@ int save_fix(eph_context* c, File& fl, int kind) {
 }
 
 // the header and the fix's state; the per-atom rows go to the context's
-// locals by tag (every local must be found)
+// locals by tag (every local must be found).  Everything is read and checked
+// before the fix is changed, so a refused file leaves the fix as it was.
 int load_fix(eph_context* c, File& fl, int want_kind) {
   char magic[8];
   int hdr[6];
@@ -170,53 +181,62 @@ int load_fix(eph_context* c, File& fl, int want_kind) {
   fl.get(e, 2);
   fl.get(dims, 4);
   if (!fl.ok) return c->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+  std::vector<double> T;
   if (c->atomic()) {
     if (dims[0] || dims[1] || dims[2]) return c->fail(EPH_ERR_FORMAT, "checkpoint grid does not match");
   } else {
     if (dims[0] != (int)c->grid->h.nx || dims[1] != (int)c->grid->h.ny || dims[2] != (int)c->grid->h.nz)
       return c->fail(EPH_ERR_FORMAT, "checkpoint grid does not match the fix's T_e grid");
-    if (int rc = get_dev(c, fl, c->grid->T.p, c->grid->h.ntotal())) return rc;
-    EPH_HIP(c, hipMemcpyAsync(c->scal.p, e, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
-    EPH_HIP(c, hipStreamSynchronize(c->stream));
+    T.resize(c->grid->h.ntotal());
+    fl.get(T.data(), T.size());
   }
-  int sz[2];
+  int sz[2] = {-1, -1};
   fl.get(sz, 2);
-  if (!fl.ok || sz[0] < 0 || sz[1] != state_cols(c)) return c->fail(EPH_ERR_FORMAT, "checkpoint per-atom block does not match");
+  if (!fl.ok) return c->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+  if (sz[0] < 0 || sz[1] != state_cols(c)) return c->fail(EPH_ERR_FORMAT, "checkpoint per-atom block does not match");
+  const long left = fl.remaining();
+  if (left < 0 || static_cast<double>(sz[0]) * (sizeof(int) + sizeof(double) * sz[1]) > static_cast<double>(left))
+    return c->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
   std::vector<int> tags(sz[0]);
   std::vector<double> st(static_cast<size_t>(sz[0]) * sz[1]);
   fl.get(tags.data(), tags.size());
   fl.get(st.data(), st.size());
   if (!fl.ok) return c->fail(EPH_ERR_FORMAT, "checkpoint file ends early");
+  // the per-atom state in the context's local order
   const int nc = sz[1];
-  if (nc > 0) {
-    if (!c->have_atoms) return c->fail(EPH_ERR_STATE, "set the atoms before loading per-atom state");
-    const int n = c->nlocal;
+  const int n = (nc > 0 && c->have_atoms) ? c->nlocal : 0;
+  if (nc > 0 && !c->have_atoms) return c->fail(EPH_ERR_STATE, "set the atoms before loading per-atom state");
+  std::vector<double> col(nc == 6 ? 6 * static_cast<size_t>(n) : 0), E(nc == 1 ? n : 0);
+  if (n) {
     std::vector<int> mine(n);
-    if (n) {
-      EPH_HIP(c, hipMemcpyAsync(mine.data(), c->tag.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
-      EPH_HIP(c, hipStreamSynchronize(c->stream));
-    }
+    EPH_HIP(c, hipMemcpyAsync(mine.data(), c->tag.p, n * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    EPH_HIP(c, hipStreamSynchronize(c->stream));
     std::unordered_map<int, int> row;
     row.reserve(tags.size() * 2);
     for (size_t r = 0; r < tags.size(); ++r) row[tags[r]] = (int)r;
-    std::vector<double> col(3 * static_cast<size_t>(n)), E(n);
     for (int i = 0; i < n; ++i) {
       const auto it = row.find(mine[i]);
       if (it == row.end()) return c->fail(EPH_ERR_FORMAT, "an atom's tag is not in the checkpoint");
       const double* s = &st[(size_t)it->second * nc];
-      if (nc == 1) E[i] = s[0];
-    }
-    if (nc == 6) {
-      for (int which = 0; which < 2; ++which) {
-        for (int i = 0; i < n; ++i) {
-          const double* s = &st[(size_t)row[mine[i]] * 6 + 3 * which];
-          for (int d = 0; d < 3; ++d) col[3 * (size_t)i + d] = s[d];
-        }
-        if (int rc = eph_set_filter_state(c, which, col.data())) return rc;
+      if (nc == 1) {
+        E[i] = s[0];
+      } else {
+        for (int which = 0; which < 2; ++which)
+          for (int d = 0; d < 3; ++d) col[3 * ((size_t)which * n + i) + d] = s[3 * which + d];
       }
-    } else if (nc == 1 && n) {
-      if (int rc = eph_set_atomic_energy(c, E.data())) return rc;
     }
+  }
+  // apply
+  if (!c->atomic()) {
+    EPH_HIP(c, hipMemcpyAsync(c->grid->T.p, T.data(), T.size() * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EPH_HIP(c, hipMemcpyAsync(c->scal.p, e, 2 * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    EPH_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  if (nc == 6 && n) {
+    for (int which = 0; which < 2; ++which)
+      if (int rc = eph_set_filter_state(c, which, col.data() + 3 * (size_t)which * n)) return rc;
+  } else if (nc == 1 && n) {
+    if (int rc = eph_set_atomic_energy(c, E.data())) return rc;
   }
   if (c->atomic()) {
     c->at->Ee = e[0];
