@@ -198,6 +198,42 @@ def test_grid_reference_goldens_bitwise(tmp_path, name):
         assert np.array_equal(g.get("T"), d["T_steps"][s]), f"step {s + 1}"
 
 
+@pytest.mark.parametrize("dims", [(13, 11, 9), (8, 8, 4), (3, 9, 5), (17, 2, 6)])
+def test_grid_mixed_cells_match_oracle(oracle, tmp_path, dims):
+    """Random grids, thin ones included, with constant, dynamic, wall and
+    temperature-dependent cells side by side and deposits every step, bitwise
+    against the oracle's EPH_FDM::solve over 6 solves"""
+    nx, ny, nz = dims
+    ncell = nx * ny * nz
+    rng = np.random.default_rng(100 * nx + 10 * ny + nz)
+    flag = rng.choice([0, 1, 2], p=[0.1, 0.75, 0.15], size=ncell)
+    tdyn = ((rng.uniform(size=ncell) < 0.3) & (flag == 1)).astype(int)
+    d = np.load(os.path.join(G, "fdm_tdyn.npz"))
+    params = str(tmp_path / "t.params")
+    inputs.write_params(params, float(d["params_dT"]), d["params_C"], d["params_kappa"])
+    box = (0.0, 1.5 * nx, -1.0, 1.2 * ny - 1.0, 2.0, 2.0 + 1.3 * nz)
+    gp = str(tmp_path / "tiles.in")
+    inputs.write_grid(gp, nx, ny, nz, box, rng.uniform(400.0, 1500.0, ncell), rng.normal(0, 1e-3, ncell),
+                      rng.uniform(0.8, 1.2, ncell), rng.uniform(1.5e-5, 3e-5, ncell),
+                      rng.uniform(0.04, 0.09, ncell), flag, tdyn, steps=2, params=params)
+    g = ea.Grid.load(gp)
+    o = oracle.FDM.load(gp)
+    g.set_dt(1e-3)
+    o.set_dt(1e-3)
+    for s in range(6):
+        n = 40
+        x = rng.uniform(box[0], box[1], n)
+        y = rng.uniform(box[2], box[3], n)
+        z = rng.uniform(box[4], box[5], n)
+        E = rng.normal(0, 1e-3, n)
+        g.insert_energy(x, y, z, E)
+        for k in range(n):
+            o.insert_energy(x[k], y[k], z[k], E[k])
+        ng, no = g.solve(), o.solve()
+        assert ng == no, (s, ng, no)
+        assert np.array_equal(g.get("T"), o.get("T")), f"solve {s + 1}"
+
+
 def test_grid_out_ref_laser(oracle):
     """Tests/EPH_FDM/Test3/Out_Ref through the HIP stencil (variable C_e, kappa)."""
     d = np.load(os.path.join(G, "fdm_outref.npz"))

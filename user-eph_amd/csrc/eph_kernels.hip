@@ -1511,6 +1511,36 @@ __device__ __forceinline__ double linear_rev(const GridView& g, double y) {
   return idx * g.E_dx + 1. / g.E_dy[idx] * (y - g.E_y[idx]);
 }
 
+// one dimension's two terms of the 7-point variable-kappa stencil
+// (eph_fdm.h:337-339, 350-352, 363-365), in the reference's order; p and q
+// are the cell's lower and upper neighbours after the wall rule
+static __device__ __forceinline__ double fdm_dim(double ddT, const double kr, const double Tr,
+                                                 const double kp, const double kq, const double Tp,
+                                                 const double Tq, const double d) {
+  ddT += (kq - kp) * (Tq - Tp) / d / d / 4.0;
+  ddT += kr * ((Tq + Tp - 2.0 * Tr) / d / d);
+  return ddT;
+}
+
+// the update of a cell from its stencil sum, then the clamp (eph_fdm.h:376-395)
+template <bool TDYN>
+static __device__ __forceinline__ double fdm_update(const GridView& g, const int r, const int fr,
+                                                    double T, const double ddT,
+                                                    const double inner_dt) {
+  if (fr == kCellDynamic) {
+    if (TDYN && g.tdyn[r] == 1) {
+      double E = linear_fwd(g, T);
+      E += (ddT + g.dT[r] + g.S[r]) / g.rho[r] * inner_dt;
+      T = linear_rev(g, E);
+    } else {
+      const double prescaler = g.rho[r] * g.C[r];
+      T += (ddT + g.dT[r] + g.S[r]) / prescaler * inner_dt;
+    }
+  }
+  if (T < 0.0) T = 0.0;
+  return T;
+}
+
 // one explicit sub-step of EPH_FDM::solve (eph_fdm.h:315-395): the 7-point
 // variable-kappa stencil on Tin, the update and the clamp, into Tout
 template <bool TDYN>
@@ -1529,37 +1559,22 @@ __device__ __forceinline__ void fdm_cell(const GridView& g, const double* __rest
     q = i < nx - 1 ? r + 1 : r - (nx - 1);
     if (g.flag[q] == kCellZeroDerivative) q = r;
     else if (g.flag[p] == kCellZeroDerivative) p = r;
-    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dx / g.dx / 4.0;
-    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dx / g.dx);
+    ddT = fdm_dim(ddT, kr, Tr, g.kappa[p], g.kappa[q], Tin[p], Tin[q], g.dx);
     // y
     p = j > 0 ? r - nx : r + (ny - 1) * nx;
     q = j < ny - 1 ? r + nx : r - (ny - 1) * nx;
     if (g.flag[q] == kCellZeroDerivative) q = r;
     else if (g.flag[p] == kCellZeroDerivative) p = r;
-    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dy / g.dy / 4.0;
-    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dy / g.dy);
+    ddT = fdm_dim(ddT, kr, Tr, g.kappa[p], g.kappa[q], Tin[p], Tin[q], g.dy);
     // z
     const int pl = nx * ny;
     p = k > 0 ? r - pl : r + (nz - 1) * pl;
     q = k < nz - 1 ? r + pl : r - (nz - 1) * pl;
     if (g.flag[q] == kCellZeroDerivative) q = r;
     else if (g.flag[p] == kCellZeroDerivative) p = r;
-    ddT += (g.kappa[q] - g.kappa[p]) * (Tin[q] - Tin[p]) / g.dz / g.dz / 4.0;
-    ddT += kr * ((Tin[q] + Tin[p] - 2.0 * Tr) / g.dz / g.dz);
+    ddT = fdm_dim(ddT, kr, Tr, g.kappa[p], g.kappa[q], Tin[p], Tin[q], g.dz);
   }
-  double T = Tin[r];
-  if (fr == kCellDynamic) {
-    if (TDYN && g.tdyn[r] == 1) {
-      double E = linear_fwd(g, T);
-      E += (ddT + g.dT[r] + g.S[r]) / g.rho[r] * inner_dt;
-      T = linear_rev(g, E);
-    } else {
-      const double prescaler = g.rho[r] * g.C[r];
-      T += (ddT + g.dT[r] + g.S[r]) / prescaler * inner_dt;
-    }
-  }
-  if (T < 0.0) T = 0.0;
-  Tout[r] = T;
+  Tout[r] = fdm_update<TDYN>(g, r, fr, Tin[r], ddT, inner_dt);
 }
 
 template <bool TDYN>
