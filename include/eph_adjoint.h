@@ -12,7 +12,11 @@
  * trajectory is a deterministic function of (beta, v0).  The forward pass
  * keeps a checkpoint every `every` steps; the backward pass recomputes each
  * segment from its checkpoint, records it, and runs the adjoint steps in
- * reverse.  Constant C_e / kappa grids only.
+ * reverse.  Grids with temperature-dependent cells (eph_fdm.h:280-287,
+ * 376-382) are differentiated through the energy-space update (the slopes of
+ * E_e(T) and its inverse) and through kappa(T), evaluated once per solve at
+ * its starting T_e; the sub-step count, a step function of the stability
+ * extrema, is held as the forward pass found it.
  */
 #ifndef EPH_ADJOINT_H
 #define EPH_ADJOINT_H

@@ -118,3 +118,61 @@ def test_beta_gradient_matches_cpu_reference(small, oracle, tmp_path):
     fd = (oracle_loss(b0 + H * d, "p") - oracle_loss(b0 - H * d, "m")) / (2 * H)
     ad = float((gb * d).sum())
     assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(small["loss"][0]), (fd, ad)
+
+
+# ------------------------------------------------- temperature-dependent grid
+TDYN = dict(SMALL, nsteps=10, every=4, md_seed=9, fix_seed=515)
+
+
+def _tdyn_grid(tmp, cells):
+    """the fix's 4^3 grid from a file: every cell dynamic, half of them with
+    C_e(T) and kappa(T) from a parameter table (the tables of the fdm_tdyn
+    golden, 10 K knots up to 4000 K)"""
+    md = ea.MD(cells, ntypes=1, seed=TDYN["md_seed"], T=TDYN["T"], pka_eV=TDYN["pka_eV"])
+    inf = md.info()
+    lo, hi = inf["boxlo"], inf["boxhi"]
+    d = np.load(os.path.join(ROOT, "tests", "golden", "fdm_tdyn.npz"))
+    params = os.path.join(tmp, "tdyn.params")
+    inputs.write_params(params, float(d["params_dT"]), d["params_C"], d["params_kappa"])
+    n = cells ** 3
+    rng = np.random.default_rng(21)
+    path = os.path.join(tmp, "tdyn.T")
+    inputs.write_grid(path, cells, cells, cells, (lo[0], hi[0], lo[1], hi[1], lo[2], hi[2]),
+                      rng.uniform(500.0, 900.0, n), 0.0, 1.0, 3e-5, 0.07, 1,
+                      (rng.uniform(size=n) < 0.5).astype(int), steps=1, params=params)
+    return path
+
+
+@pytest.fixture(scope="module")
+def tdyn(tmp_path_factory):
+    tmp = str(tmp_path_factory.mktemp("adjt"))
+    spec = dict(TDYN, tmpdir=tmp, T_infile=_tdyn_grid(tmp, TDYN["cells"]))
+    md, fix, adj, meta = build_trajectory(spec)
+    loss0 = adj.forward()
+    assert fix.grid().get("C").std() > 0.0  # C_e(T) evaluated per cell
+    gb, gv = adj.backward()
+    yield dict(adj=adj, loss=loss0, gb=gb, gv=gv, fix=fix)
+    adj.close()
+
+
+def test_tdyn_grid_gradients_match_central_differences(tdyn):
+    """beta and v0 gradients through temperature-dependent C_e / kappa cells
+    (energy-space update, kappa(T) at each solve's start) against central
+    differences of the device run"""
+    adj, gb, gv, loss = tdyn["adj"], tdyn["gb"], tdyn["gv"], tdyn["loss"]
+    rng = np.random.default_rng(31)
+    b0, v0 = adj.beta0(), adj.v0()
+    scale = np.abs(v0).mean()
+    for kind in ("beta", "v0"):
+        if kind == "beta":
+            d = rng.standard_normal(b0.shape) * b0
+            fd = _central(lambda b: adj.forward(beta=b)[0], b0, d, H)
+            ad = float((gb * d).sum())
+        else:
+            d = rng.standard_normal(v0.shape) * scale
+            fd = _central(lambda v: adj.forward(v0=v)[0], v0, d, H)
+            ad = float((gv * d).sum())
+        assert abs(fd - ad) <= 1e-5 * abs(ad) + 1e-9 * abs(loss[0]), (kind, fd, ad)
+    # the grid term carries the T_e dependence through the solves
+    assert loss[2] > 0.0
+    adj.forward()

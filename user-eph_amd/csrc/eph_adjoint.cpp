@@ -119,7 +119,7 @@ struct eph_adjoint {
 
   // adjoint accumulators
   DevArray<double4> xb, vhb, wb, vnb, fEb, fRb, ftb;
-  DevArray<double> ab, rb, Tb, Tb2, dTb, Gb, Tsub;
+  DevArray<double> ab, rb, Tb, Tb2, dTb, Gb, Tsub, kb;
   bool have_forward = false;
 
   int fail(int code, const std::string& m) { err = m; return code; }
@@ -277,21 +277,40 @@ int eph_adjoint::backward_step(Rec& r, double*) {
   gv.dT = r.DT.p;
   double* cur = Tb.p;
   if (ctx->flags & EPH_FDM) {
+    eph_grid* grid = ctx->grid;
+    const bool tdyn = grid->any_tdyn;
     double rr = 0, inner = 0;
     unsigned S = 0;
-    ctx->grid->refresh_minmax();
-    ctx->grid->substeps(&rr, &S, &inner);
+    if (tdyn) {
+      // this step's solve: C_e(T), kappa(T) at its starting T_e, then the
+      // stability extrema and the sub-step count from them
+      GridView g0 = gv;
+      g0.T = r.TB.p;
+      EPH_HIP(this, launch_grid_tdyn_update(g0, s));
+      EPH_HIP(this, launch_grid_minmax(g0, grid->minmax.p, nullptr, s));
+      double mm[3];
+      EPH_HIP(this, hipMemcpyAsync(mm, grid->minmax.p, sizeof mm, hipMemcpyDeviceToHost, s));
+      EPH_HIP(this, hipStreamSynchronize(s));
+      grid->cmin = mm[0]; grid->rmin = mm[1]; grid->kmax = mm[2];
+      EPH_HIP(this, kb.reserve(G));
+      EPH_HIP(this, launch_fill_double(kb.p, G, 0.0, s));
+    } else {
+      grid->refresh_minmax();
+    }
+    grid->substeps(&rr, &S, &inner);
     EPH_HIP(this, Tsub.reserve((size_t)(S + 1) * G));
     HCPY(Tsub.p, r.TB.p, G * sizeof(double));
     for (unsigned k = 0; k < S; ++k)
       EPH_HIP(this, launch_fdm_substep(gv, Tsub.p + (size_t)k * G, Tsub.p + (size_t)(k + 1) * G,
-                                       inner, false, s));
+                                       inner, tdyn, s));
     double* other = Tb2.p;
     for (int k = (int)S - 1; k >= 0; --k) {
       EPH_HIP(this, launch_fill_double(other, G, 0.0, s));
-      EPH_HIP(this, launch_adj_fdm(gv, Tsub.p + (size_t)k * G, cur, other, dTb.p, inner, s));
+      EPH_HIP(this, launch_adj_fdm(gv, Tsub.p + (size_t)k * G, cur, other, dTb.p, tdyn ? kb.p : nullptr,
+                                   inner, s));
       std::swap(cur, other);
     }
+    if (tdyn) EPH_HIP(this, launch_adj_kappa_T(gv, r.TB.p, kb.p, cur, s));
   }
   b.Tb = cur;  // dL/dT_e before this step's solve; the force sweep adds its part
   AdjStepView sv;
@@ -340,10 +359,6 @@ int eph_adjoint_create(eph_adjoint** out, eph_md* md, eph_context* ctx, int nste
   if (!ctx->have_atoms || ctx->nlocal != md->nlocal || !md->device_rebuild) {
     g_adj_error = "the harness must be attached to the fix (device rebuilds on)";
     return EPH_ERR_STATE;
-  }
-  if (ctx->grid->any_tdyn) {
-    g_adj_error = "temperature-dependent C_e / kappa grids are not differentiated";
-    return EPH_ERR_UNSUPPORTED;
   }
   if (ctx->model != 4 || ctx->colour >= 2) {
     g_adj_error = "only the PRL model of the eph and eph/coloured styles is differentiated";

@@ -34,12 +34,18 @@ inline int nblk(long n) { return static_cast<int>((n + kBlock - 1) / kBlock); }
 }  // namespace
 
 // ------------------------------------------------------------------ FDM
-// one sub-step of k_fdm_substep (constant C_e, kappa) in reverse:
-// Tb_in += (dT_out/dT_in)^T Tb_out, dTb += (dT_out/d dT)^T Tb_out
+// one sub-step of k_fdm_substep in reverse:
+// Tb_in += (dT_out/dT_in)^T Tb_out, dTb += (dT_out/d dT)^T Tb_out, and with
+// temperature-dependent cells (TDYN) kb += (dT_out/d kappa)^T Tb_out.  A
+// temperature-dependent cell updates in energy space, T_out =
+// E^-1(E(T_in) + (ddT + dT + S) / rho dt_in), whose derivatives are the
+// slopes of the two piecewise-linear lookups; the others update
+// T_out = T_in + (ddT + dT + S) / (rho C) dt_in.
+template <bool TDYN>
 __global__ __launch_bounds__(kBlock) void k_adj_fdm(GridView g, const double* __restrict__ Tin,
                                                     const double* __restrict__ Tb_out,
-                                                    double* Tb_in, double* dTb, double inner_dt,
-                                                    int G) {
+                                                    double* Tb_in, double* dTb, double* kb,
+                                                    double inner_dt, int G) {
   const int r = blockIdx.x * kBlock + threadIdx.x;
   if (r >= G) return;
   const double tb = Tb_out[r];
@@ -76,13 +82,22 @@ __global__ __launch_bounds__(kBlock) void k_adj_fdm(GridView g, const double* __
     }
   }
   double T = Tin[r];
-  double kfac = 0.0;
+  double kfac = 0.0;  // dT_out / d ddT (= dT_out / d dT)
+  double sfac = 1.0;  // dT_out / dT_in through the cell's own update
   if (fr == kCellDynamic) {
-    kfac = inner_dt / (g.rho[r] * g.C[r]);
-    T += (ddT + g.dT[r] + g.S[r]) / (g.rho[r] * g.C[r]) * inner_dt;
+    if (TDYN && g.tdyn[r] == 1) {
+      const double E = linear_fwd(g, T) + (ddT + g.dT[r] + g.S[r]) / g.rho[r] * inner_dt;
+      const double inv = linear_rev_slope(g, E);
+      sfac = inv * linear_fwd_slope(g, T);
+      kfac = inv * inner_dt / g.rho[r];
+      T = linear_rev(g, E);
+    } else {
+      kfac = inner_dt / (g.rho[r] * g.C[r]);
+      T += (ddT + g.dT[r] + g.S[r]) / (g.rho[r] * g.C[r]) * inner_dt;
+    }
   }
   if (T < 0.0 || tb == 0.0) return;  // clamped to 0 K: no dependence
-  atomicAdd(&Tb_in[r], tb);
+  atomicAdd(&Tb_in[r], tb * sfac);
   if (fr != kCellDynamic) return;
   atomicAdd(&dTb[r], tb * kfac);
   const double kr = g.kappa[r];
@@ -94,7 +109,25 @@ __global__ __launch_bounds__(kBlock) void k_adj_fdm(GridView g, const double* __
     atomicAdd(&Tb_in[Q], tb * kfac * cq);
     atomicAdd(&Tb_in[P], tb * kfac * cp);
     atomicAdd(&Tb_in[r], tb * kfac * (-2.0 * kr / h2));
+    if (TDYN) {
+      // ddT's dependence on kappa at q, p and the cell itself
+      const double dq = (Tin[Q] - Tin[P]) / h2 / 4.0;
+      atomicAdd(&kb[Q], tb * kfac * dq);
+      atomicAdd(&kb[P], -tb * kfac * dq);
+      atomicAdd(&kb[r], tb * kfac * ((Tin[Q] + Tin[P] - 2.0 * Tin[r]) / h2));
+    }
   }
+}
+
+// kappa(T) of the temperature-dependent cells is evaluated once per solve at
+// the solve's starting T_e (eph_fdm.h:280-287): its gradient, gathered over
+// the sub-steps in kb, goes to that T_e through the spline's slope
+__global__ __launch_bounds__(kBlock) void k_adj_kappa_T(GridView g, const double* __restrict__ T0,
+                                                        const double* __restrict__ kb, double* Tb,
+                                                        int G) {
+  const int c = blockIdx.x * kBlock + threadIdx.x;
+  if (c >= G || !g.tdyn[c] || kb[c] == 0.0) return;
+  Tb[c] += kb[c] * cubic_slope(g.kappa_T, g.n_kT, g.inv_dT_k, T0[c]);
 }
 
 // ------------------------------------------------------------- deposit
@@ -457,9 +490,17 @@ __global__ __launch_bounds__(kBlock) void k_adj_initial(AtomsView a, StepParams 
 
 // ------------------------------------------------------------ wrappers
 hipError_t launch_adj_fdm(const GridView& g, const double* Tin, const double* Tb_out, double* Tb_in,
-                          double* dTb, double inner_dt, hipStream_t s) {
+                          double* dTb, double* kb, double inner_dt, hipStream_t s) {
   const int G = g.nx * g.ny * g.nz;
-  k_adj_fdm<<<nblk(G), kBlock, 0, s>>>(g, Tin, Tb_out, Tb_in, dTb, inner_dt, G);
+  if (kb) k_adj_fdm<true><<<nblk(G), kBlock, 0, s>>>(g, Tin, Tb_out, Tb_in, dTb, kb, inner_dt, G);
+  else k_adj_fdm<false><<<nblk(G), kBlock, 0, s>>>(g, Tin, Tb_out, Tb_in, dTb, nullptr, inner_dt, G);
+  return hipGetLastError();
+}
+
+hipError_t launch_adj_kappa_T(const GridView& g, const double* T0, const double* kb, double* Tb,
+                              hipStream_t s) {
+  const int G = g.nx * g.ny * g.nz;
+  k_adj_kappa_T<<<nblk(G), kBlock, 0, s>>>(g, T0, kb, Tb, G);
   return hipGetLastError();
 }
 

@@ -41,8 +41,13 @@ struct AdjStepFlags {
   double lj1 = 0, lj2 = 0, ljrc2 = 0;
 };
 
+// kb (dL/d kappa of the step's solve) is gathered only for grids with
+// temperature-dependent cells; nullptr otherwise
 hipError_t launch_adj_fdm(const GridView& g, const double* Tin, const double* Tb_out, double* Tb_in,
-                          double* dTb, double inner_dt, hipStream_t s);
+                          double* dTb, double* kb, double inner_dt, hipStream_t s);
+// Tb += kb dkappa/dT at the solve's starting T_e (T0), temperature-dependent cells
+hipError_t launch_adj_kappa_T(const GridView& g, const double* T0, const double* kb, double* Tb,
+                              hipStream_t s);
 hipError_t launch_adj_step(const AdjStepView& sv, const AtomsView& a, const TablesView& t,
                            const GridView& g, const StepParams& p, const AdjStepFlags& f,
                            const double* dTb, const AdjBuffers& b, hipStream_t s);

@@ -68,4 +68,59 @@ static __device__ __forceinline__ T block_sum(T v, T* sh) {
 }
 
 
+// E_e(T) of a temperature-dependent cell: EPH_Linear::operator()
+// (eph_linear.h:42-49); the top knot, whose slope the reference reads past
+// its table, returns the knot value
+static __device__ __forceinline__ double linear_fwd(const GridView& g, double x) {
+  if (!(x >= 0.0)) return 0.0;
+  const double q = x / g.E_dx;
+  if (q >= static_cast<double>(g.n_E)) return 0.0;
+  const int idx = static_cast<int>(q);
+  const double delta = x - idx * g.E_dx;
+  if (idx >= g.n_E - 1) return g.E_y[idx];
+  return g.E_y[idx] + g.E_dy[idx] * delta;
+}
+
+// the slope of linear_fwd at x (0 where it is constant)
+static __device__ __forceinline__ double linear_fwd_slope(const GridView& g, double x) {
+  if (!(x >= 0.0)) return 0.0;
+  const double q = x / g.E_dx;
+  if (q >= static_cast<double>(g.n_E)) return 0.0;
+  const int idx = static_cast<int>(q);
+  return idx >= g.n_E - 1 ? 0.0 : g.E_dy[idx];
+}
+
+// the segment of E_e holding y (EPH_Linear::reverse_lookup's upper_bound,
+// then one step back), or -1 where the lookup returns 0
+static __device__ __forceinline__ int linear_rev_segment(const GridView& g, double y) {
+  int lo = 0, hi = g.n_E;
+  while (lo < hi) {
+    const int mid = (lo + hi) >> 1;
+    if (y < g.E_y[mid]) hi = mid; else lo = mid + 1;
+  }
+  return (lo >= g.n_E || lo == 0) ? -1 : lo - 1;
+}
+
+// T of energy y: EPH_Linear::reverse_lookup (eph_linear.h:52-61)
+static __device__ __forceinline__ double linear_rev(const GridView& g, double y) {
+  const int idx = linear_rev_segment(g, y);
+  if (idx < 0) return 0.0;
+  return idx * g.E_dx + 1. / g.E_dy[idx] * (y - g.E_y[idx]);
+}
+
+// the slope of linear_rev at y
+static __device__ __forceinline__ double linear_rev_slope(const GridView& g, double y) {
+  const int idx = linear_rev_segment(g, y);
+  return idx < 0 ? 0.0 : 1. / g.E_dy[idx];
+}
+
+// d/dx of cubic_at (the same interval, absolute x)
+static __device__ __forceinline__ double cubic_slope(const double4* __restrict__ tab, int n, double inv,
+                                                     double x) {
+  int i = static_cast<int>(x * inv);
+  i = i < 0 ? 0 : (i < n - 1 ? i : n - 1);
+  const double4 c = tab[i];
+  return c.y + x * (2.0 * c.z + x * 3.0 * c.w);
+}
+
 }  // namespace eph

@@ -1487,30 +1487,6 @@ __global__ __launch_bounds__(1024) void k_grid_minmax(GridView g, int n, double*
   if (threadIdx.x == 0) { out3[0] = s0[0]; out3[1] = s1[0]; out3[2] = s2[0]; }
 }
 
-__device__ __forceinline__ double linear_fwd(const GridView& g, double x) {
-  // EPH_Linear::operator() (eph_linear.h:42-49); the top knot, whose slope
-  // the reference reads past its table, returns the knot value
-  if (!(x >= 0.0)) return 0.0;
-  const double q = x / g.E_dx;
-  if (q >= static_cast<double>(g.n_E)) return 0.0;
-  const int idx = static_cast<int>(q);
-  const double delta = x - idx * g.E_dx;
-  if (idx >= g.n_E - 1) return g.E_y[idx];
-  return g.E_y[idx] + g.E_dy[idx] * delta;
-}
-
-__device__ __forceinline__ double linear_rev(const GridView& g, double y) {
-  // EPH_Linear::reverse_lookup (eph_linear.h:52-61): upper_bound, step back
-  int lo = 0, hi = g.n_E;
-  while (lo < hi) {
-    const int mid = (lo + hi) >> 1;
-    if (y < g.E_y[mid]) hi = mid; else lo = mid + 1;
-  }
-  if (lo >= g.n_E || lo == 0) return 0.0;
-  const int idx = lo - 1;
-  return idx * g.E_dx + 1. / g.E_dy[idx] * (y - g.E_y[idx]);
-}
-
 // one dimension's two terms of the 7-point variable-kappa stencil
 // (eph_fdm.h:337-339, 350-352, 363-365), in the reference's order; p and q
 // are the cell's lower and upper neighbours after the wall rule

@@ -31,7 +31,7 @@ def build_trajectory(spec):
     inf = md.info()
     g = spec.get("grid", cells)
     args = inputs.fix_args(beta, names, seed=spec.get("fix_seed", 4711), flags=spec.get("flags", "7"),
-                           grid=(g, g, g))
+                           grid=(g, g, g), T_infile=spec.get("T_infile", "NULL"))
     fix = ea.FixEPH(args, 1, inf["boxlo"], inf["boxhi"], spec["dt"], inputs.BOLTZ, inputs.FTM2V,
                     masses=[inputs.MASS_CU])
     fix.init()
