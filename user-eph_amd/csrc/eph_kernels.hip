@@ -574,26 +574,24 @@ __global__ __launch_bounds__(kBlock) void k_pack_pv(AtomsView a, WorkView w, int
 // (fe) and random (fr) sums: one definition for every form of the sweep, so
 // that all of them round alike.
 template <bool FRIC, bool RAND, bool MULTI>
-__device__ __forceinline__ void prl_pair(const double4& pi, const double4& rwi, const double4& xai,
-                                         const int ei, const double4& pj, const double4& rwj,
-                                         const double4& xaj, const TablesView& t,
-                                         const double4* __restrict__ rtab, double* fe, double* fr) {
+__device__ __forceinline__ void prl_pair_terms(const double4& pi, const double4& rwi, const double4& xai,
+                                               const double4& pj, const double4& rwj, const double4& xaj,
+                                               const double r2, const double4& cji, const double4& cij,
+                                               double* te, double* tr) {
   const double rho_i = rwi.w, alpha_i = xai.w;
   const double rho_j = rwj.w, alpha_j = xaj.w;
   const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
-  const double r2 = ex * ex + ey * ey + ez * ez;
-  const int ej = MULTI ? static_cast<int>(pj.w) : 0;
-  const double vr_ji = rho_r2_at(rtab, t, ej, r2);
-  const double vr_ij = MULTI ? rho_r2_at(rtab, t, ei, r2) : vr_ji;
+  const double vr_ji = cji.x + r2 * (cji.y + r2 * (cji.z + r2 * cji.w));
+  const double vr_ij = MULTI ? cij.x + r2 * (cij.y + r2 * (cij.z + r2 * cij.w)) : vr_ji;
   if (FRIC) {
     const double ev1 = ex * rwi.x + ey * rwi.y + ez * rwi.z;
     const double var1 = alpha_i * vr_ji * ev1 / (rho_i * r2);
     const double ev2 = ex * rwj.x + ey * rwj.y + ez * rwj.z;
     const double var2 = alpha_j * vr_ij * ev2 / (rho_j * r2);
     const double dvar = var1 - var2;
-    fe[0] -= dvar * ex;
-    fe[1] -= dvar * ey;
-    fe[2] -= dvar * ez;
+    te[0] = dvar * ex;
+    te[1] = dvar * ey;
+    te[2] = dvar * ez;
   }
   if (RAND) {
     const double ex1 = ex * xai.x + ey * xai.y + ez * xai.z;
@@ -601,10 +599,41 @@ __device__ __forceinline__ void prl_pair(const double4& pi, const double4& rwi, 
     const double ex2 = ex * xaj.x + ey * xaj.y + ez * xaj.z;
     const double var2 = alpha_j * vr_ij * ex2 / (rho_j * r2);
     const double dvar = var1 - var2;
-    fr[0] += dvar * ex;
-    fr[1] += dvar * ey;
-    fr[2] += dvar * ez;
+    tr[0] = dvar * ex;
+    tr[1] = dvar * ey;
+    tr[2] = dvar * ez;
   }
+}
+
+// the pair's terms taken into the sums: f_EPH -= dvar e, f_RNG += dvar e
+template <bool FRIC, bool RAND, bool MULTI>
+__device__ __forceinline__ void prl_pair_coef(const double4& pi, const double4& rwi, const double4& xai,
+                                              const double4& pj, const double4& rwj, const double4& xaj,
+                                              const double r2, const double4& cji, const double4& cij,
+                                              double* fe, double* fr) {
+  double te[3], tr[3];
+  prl_pair_terms<FRIC, RAND, MULTI>(pi, rwi, xai, pj, rwj, xaj, r2, cji, cij, te, tr);
+#pragma unroll
+  for (int d = 0; d < 3; ++d) {
+    if (FRIC) fe[d] -= te[d];
+    if (RAND) fr[d] += tr[d];
+  }
+}
+
+// the same with the coefficients looked up here (rho_r2_at)
+template <bool FRIC, bool RAND, bool MULTI>
+__device__ __forceinline__ void prl_pair(const double4& pi, const double4& rwi, const double4& xai,
+                                         const int ei, const double4& pj, const double4& rwj,
+                                         const double4& xaj, const TablesView& t,
+                                         const double4* __restrict__ rtab, double* fe, double* fr) {
+  const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+  const double r2 = ex * ex + ey * ey + ez * ez;
+  int kn = static_cast<int>(r2 * t.inv_dr2);
+  kn = kn < 0 ? 0 : (kn < t.n_rho - 1 ? kn : t.n_rho - 1);
+  const int ej = MULTI ? static_cast<int>(pj.w) : 0;
+  const double4 cji = rtab[kn * t.n_elem + ej];
+  const double4 cij = MULTI ? rtab[kn * t.n_elem + ei] : cji;
+  prl_pair_coef<FRIC, RAND, MULTI>(pi, rwi, xai, pj, rwj, xaj, r2, cji, cij, fe, fr);
 }
 
 // After the sums: the sqrt(T_e) factor of the random force, the coloured/exp
@@ -768,44 +797,98 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
   const int q8 = lane >> 3, part = ((lane & 7) - q8) & 7;
   const int rot = lane & 7;
   const double2* mine = stage + 72 * (lane >> 3) + 8 * (lane & 7);
-  // kCoopF entries per iteration, the records of entry k + e at
-  // stage + kCoopStage e
-  constexpr int U = kCoopF;
-  int jj[U];
+  static_assert(kCoopF == 1, "one entry per iteration");
+  // Two-stage pipeline.  Iteration k waits for everything in flight (the
+  // coefficients of entry k, the records of entry k + 1), reads entry k + 1's
+  // records from LDS, sends out its coefficient loads and the copies of entry
+  // k + 2, and only then computes pair k, whose inputs are all in registers:
+  // the pair's arithmetic runs while the next loads are in flight, and no
+  // wait depends on the order in which loads of different kinds return.
+  // Loads past the wave's longest list fetch record 0 and knot 0, which
+  // nothing uses.
+  constexpr unsigned kWaitVm0 = 0x0F70;  // s_waitcnt vmcnt(0), expcnt and lgkmcnt untouched
+  auto row = [&](int r) { return r < nmax ? r : (nmax > 0 ? nmax - 1 : 0); };
+  auto knot = [&](double r2, bool act) {
+    int kn = static_cast<int>(r2 * t.inv_dr2);
+    kn = kn < 0 ? 0 : (kn < t.n_rho - 1 ? kn : t.n_rho - 1);
+    return act ? kn : 0;
+  };
+  auto read_entry = [&](double4& pj, double4& rwj, double4& xaj) {
+    double2 sv[6];
 #pragma unroll
-  for (int e = 0; e < U; ++e) jj[e] = U + e < na ? w.act[(U + e) * stride + i] : 0;
+    for (int c = 0; c < 6; ++c) sv[c] = mine[(c + rot) & 7];
+    __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+    pj = make_double4(sv[0].x, sv[0].y, sv[1].x, sv[1].y);
+    rwj = make_double4(sv[2].x, sv[2].y, sv[3].x, sv[3].y);
+    xaj = make_double4(sv[4].x, sv[4].y, sv[5].x, sv[5].y);
+  };
+  auto dist2 = [&](const double4& pj) {
+    const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+    return ex * ex + ey * ey + ez * ez;
+  };
+  // the state of one entry: its records, coefficients, r^2 and whether it counts
+  struct Entry {
+    double4 pj, rwj, xaj, cji, cij;
+    double r2;
+    bool act;
+  };
+  // read entry k's records from the stage and send out its coefficient loads
+  auto take = [&](Entry& e, int k) {
+    read_entry(e.pj, e.rwj, e.xaj);
+    wave_lds_sync();
+    e.r2 = dist2(e.pj);
+    e.act = k < na && e.rwj.w > 0.0;
+    const int kn = knot(e.r2, e.act);
+    e.cji = rtab[kn * t.n_elem + (MULTI ? static_cast<int>(e.pj.w) : 0)];
+    e.cij = MULTI ? rtab[kn * t.n_elem + ei] : e.cji;
+  };
+  // pair k's terms into the sums (only an active lane's)
+  auto add = [&](const Entry& e) {
+    double te[3], tr[3];
+    prl_pair_terms<FRIC, RAND, MULTI>(pi, rwi, xai, e.pj, e.rwj, e.xaj, e.r2, e.cji, e.cij, te, tr);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      if (FRIC) fe[d] = e.act ? fe[d] - te[d] : fe[d];
+      if (RAND) fr[d] = e.act ? fr[d] + tr[d] : fr[d];
+    }
+  };
+  int jraw = 0;
+  // one iteration: entry k + 1 taken, entry k + 2's copies and entry k + 3's
+  // index sent, then pair k (cur) computed while they are in flight
+  auto iter = [&](int k, Entry& cur, Entry& nxt) {
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
+    wave_lds_sync();
+    take(nxt, k + 1);
+    coop_fetch(recs, k + 2 < na ? jraw : 0, q8, part, stage);
+    jraw = w.act[row(k + 3) * stride + i];
+    // the loads above go out first; pair k's arithmetic (whose inputs are
+    // all in registers, its loads having completed at the top) comes after
+    // them, so it runs while they are in flight
+    asm volatile("" : "+v"(cur.pj.x), "+v"(cur.pj.y), "+v"(cur.pj.z), "+v"(cur.r2) :: "memory");
+    add(cur);
+    // ... and ends before the next iteration's wait
+    asm volatile("" : "+v"(fe[0]), "+v"(fe[1]), "+v"(fe[2]), "+v"(fr[0]), "+v"(fr[1]), "+v"(fr[2]));
+  };
+  Entry e0, e1;
   if (nmax > 0) {
-#pragma unroll
-    for (int e = 0; e < U; ++e)
-      coop_fetch(recs, e < na ? w.act[e * stride + i] : 0, q8, part, stage + kCoopStage * e);
-  }
-  for (int k = 0; k < nmax; k += U) {
-    // entries k.. have landed; read this lane's records, then let the copies
-    // of the next entries overwrite the stage
-    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    coop_fetch(recs, na > 0 ? w.act[i] : 0, q8, part, stage);
+    jraw = w.act[row(1) * stride + i];
+    __builtin_amdgcn_s_waitcnt(kWaitVm0);
     wave_lds_sync();
-    double2 sv[U][6];
-#pragma unroll
-    for (int e = 0; e < U; ++e)
-#pragma unroll
-      for (int c = 0; c < 6; ++c) sv[e][c] = mine[kCoopStage * e + ((c + rot) & 7)];
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    wave_lds_sync();
-    if (k + U < nmax) {
-#pragma unroll
-      for (int e = 0; e < U; ++e) coop_fetch(recs, jj[e], q8, part, stage + kCoopStage * e);
-#pragma unroll
-      for (int e = 0; e < U; ++e) jj[e] = k + 2 * U + e < na ? w.act[(k + 2 * U + e) * stride + i] : 0;
-    }
-#pragma unroll
-    for (int e = 0; e < U; ++e) {
-      const double4 pj = make_double4(sv[e][0].x, sv[e][0].y, sv[e][1].x, sv[e][1].y);
-      const double4 rwj = make_double4(sv[e][2].x, sv[e][2].y, sv[e][3].x, sv[e][3].y);
-      const double4 xaj = make_double4(sv[e][4].x, sv[e][4].y, sv[e][5].x, sv[e][5].y);
-      if (k + e < na && rwj.w > 0.0)
-        prl_pair<FRIC, RAND, MULTI>(pi, rwi, xai, ei, pj, rwj, xaj, t, rtab, fe, fr);
-    }
+    take(e0, 0);
+    coop_fetch(recs, 1 < na ? jraw : 0, q8, part, stage);
+    jraw = w.act[row(2) * stride + i];
   }
+  // two entries per trip, the two Entry slots taking turns (no copies of
+  // registers whose loads are still in flight)
+  int k = 0;
+  for (; k + 1 < nmax; k += 2) {
+    iter(k, e0, e1);
+    iter(k + 1, e1, e0);
+  }
+  if (k < nmax) iter(k, e0, e1);
+  // the last copies land before the wave may leave
+  __builtin_amdgcn_s_waitcnt(kWaitVm0);
   if (valid) prl_finish<FRIC, RAND>(i, on, pi, fe, fr, a, g, p, w);
 }
 
