@@ -362,6 +362,9 @@ def main():
                     help="skip the one-replica-at-a-time comparison of the ensemble line")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="steps of the trajectory-gradient line (0 = skip it)")
+    ap.add_argument("--per-step", type=int, default=0, metavar="N",
+                    help="after the timed steps, N more steps one call each, with each step's "
+                         "sweep times (HIP events) and candidate-list statistics")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
@@ -611,6 +614,28 @@ def main():
                     "not the value, which keeps the atoms resident in HBM"}
         log(f"host-array path: {rate:.3e} atom-steps/s")
         snap_host = None
+    per_step = []
+    for _ in range(args.per_step):
+        # one step per call: the sweeps of this step alone, and which waves
+        # walked their candidates (the rest scanned their full lists)
+        fix.timing(True)
+        try:
+            r = advance(1, True)
+        except ea.EphError as e:  # e.g. the unstable stencil's grid turned non-finite
+            fix.timing(False)
+            per_step.append({"step": step, "stopped": str(e)})
+            break
+        rp = fix.timing_report()
+        fix.timing(False)
+        cs = fix.candidate_stats()
+        per_step.append({"step": step - 1, "rebuilt": bool(r["rebuilds"]),
+                         "density_w_ms": rp.get("density_w", (float("nan"), 0))[0],
+                         "prl_forces_ms": rp.get("prl_forces", (float("nan"), 0))[0],
+                         "full_list_waves": cs["full_waves"], "waves": cs["waves"],
+                         "moved_atoms": cs["moved"], "candidates_retaken": cs["retaken"]})
+        log(f"step {step - 1}: {per_step[-1]}")
+    if per_step:
+        out["per_step"] = per_step
     if args.ensemble_replicas > 0:
         fix.close()
         md.close()
