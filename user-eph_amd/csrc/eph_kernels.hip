@@ -1700,34 +1700,69 @@ __global__ __launch_bounds__(kBlock) void k_peratom(AtomsView a, TablesView t, W
 }
 
 // ---------------------------------------------------------- harness
-// pair lj/cut, full list, Newton off: f_i = sum_j fpair (x_i - x_j)
+// pair lj/cut, full list, Newton off: f_i = sum_j fpair (x_i - x_j), summed
+// in list order.  The neighbours' positions come through LDS as in the
+// density sweep's rho loop (coop_fetch_pos, kLjU entries per iteration, the
+// next entries' copies in flight while these are summed); the whole wave runs
+// to its longest list.
+constexpr int kLjU = 4;
 __global__ __launch_bounds__(kBlock) void k_lj(AtomsView a, const int* __restrict__ nbr,
                                                const int* __restrict__ nn, int stride, double lj1,
                                                double lj2, double rc2) {
-  const int i = xcd_block() * kBlock + threadIdx.x;
-  if (i >= a.nlocal) return;
+  __shared__ double2 s_lj[(kBlock / 64) * kLjU * 128];
+  const int base = xcd_block() * kBlock;
+  if (base + static_cast<int>(threadIdx.x & ~63u) >= a.nlocal) return;  // whole wave past the end
+  const int i_lane = base + threadIdx.x;
+  const bool valid = i_lane < a.nlocal;
+  const int i = valid ? i_lane : 0;
+  const int lane = threadIdx.x & 63;
+  double2* stage = s_lj + (threadIdx.x >> 6) * kLjU * 128;
+  const double2* __restrict__ pos2 = reinterpret_cast<const double2*>(a.pos);
   const double4 pi = a.pos[i];
   double fx = 0.0, fy = 0.0, fz = 0.0;
-  const int n = nn[i];
-  int jn = n > 0 ? nbr[i] : 0;
-  double4 pn = n > 0 ? a.pos[jn] : pi;
-  for (int k = 0; k < n; ++k) {
-    const double4 pj = pn;
-    if (k + 1 < n) {
-      jn = nbr[(k + 1) * stride + i];
-      pn = a.pos[jn];
+  const int n = valid ? nn[i] : 0;
+  const int nmax = wave_max(n);
+  constexpr int U = kLjU;
+  int jj[U], nx[U];
+#pragma unroll
+  for (int e = 0; e < U; ++e) jj[e] = e < n ? nbr[e * stride + i] : 0;
+#pragma unroll
+  for (int e = 0; e < U; ++e) nx[e] = U + e < n ? nbr[(U + e) * stride + i] : 0;
+  if (nmax > 0) {
+#pragma unroll
+    for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, jj[e], lane, stage + 128 * e);
+  }
+  for (int k = 0; k < nmax; k += U) {
+    asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
+    wave_lds_sync();
+    double4 pq[U];
+#pragma unroll
+    for (int e = 0; e < U; ++e) pq[e] = coop_read_pos(stage + 128 * e, lane);
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    wave_lds_sync();
+    if (k + U < nmax) {
+#pragma unroll
+      for (int e = 0; e < U; ++e) coop_fetch_pos(pos2, nx[e], lane, stage + 128 * e);
+#pragma unroll
+      for (int e = 0; e < U; ++e) nx[e] = k + 2 * U + e < n ? nbr[(k + 2 * U + e) * stride + i] : 0;
     }
-    const double dx = pi.x - pj.x, dy = pi.y - pj.y, dz = pi.z - pj.z;
-    const double r2 = dx * dx + dy * dy + dz * dz;
-    if (r2 < rc2) {
-      const double r2inv = 1.0 / r2;
-      const double r6inv = r2inv * r2inv * r2inv;
-      const double fpair = r6inv * (lj1 * r6inv - lj2) * r2inv;
-      fx += dx * fpair;
-      fy += dy * fpair;
-      fz += dz * fpair;
+#pragma unroll
+    for (int e = 0; e < U; ++e) {
+      if (k + e < n) {
+        const double dx = pi.x - pq[e].x, dy = pi.y - pq[e].y, dz = pi.z - pq[e].z;
+        const double r2 = dx * dx + dy * dy + dz * dz;
+        if (r2 < rc2) {
+          const double r2inv = 1.0 / r2;
+          const double r6inv = r2inv * r2inv * r2inv;
+          const double fpair = r6inv * (lj1 * r6inv - lj2) * r2inv;
+          fx += dx * fpair;
+          fy += dy * fpair;
+          fz += dz * fpair;
+        }
+      }
     }
   }
+  if (!valid) return;
   const int fs = a.fstride;
   a.f[i] = fx;
   a.f[fs + i] = fy;
