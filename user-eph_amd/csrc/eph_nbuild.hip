@@ -194,6 +194,27 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
   const int bx = bin_coord(pi.x, glo[0], b.bs, nb[0]);
   const int by = bin_coord(pi.y, glo[1], b.bs, nb[1]);
   const int bz = bin_coord(pi.z, glo[2], b.bs, nb[2]);
+  // Squared distance from the atom to each bin slab of the 5 x 5 x 5 stencil,
+  // per dimension.  A bin whose box lies farther than the larger cutoff
+  // holds no atom of either list and is skipped, which leaves the lists (and
+  // their order) unchanged.  Edge bins also hold the atoms clamped into them,
+  // so they are never skipped; the margin covers the rounding of the slab
+  // bounds.
+  double sx[5], sy[5], sz[5];
+  {
+    const double pc[3] = {pi.x, pi.y, pi.z};
+    const int bc[3] = {bx, by, bz};
+    double* sd[3] = {sx, sy, sz};
+    for (int d = 0; d < 3; ++d) {
+      for (int o = 0; o < 5; ++o) {
+        const int c = bc[d] + o - 2;
+        const double lo = glo[d] + c * b.bs, hi = lo + b.bs;
+        const double g = (c <= 0 || c >= nb[d] - 1) ? 0.0 : (pc[d] < lo ? lo - pc[d] : (pc[d] > hi ? pc[d] - hi : 0.0));
+        sd[d][o] = g * g;
+      }
+    }
+  }
+  const double reach2 = b.cg * b.cg * (1.0 + 1e-9);
   int nf = 0, nl = 0, nc = 0;
   for (int dz = -2; dz <= 2; ++dz) {
     const int z = bz + dz;
@@ -204,6 +225,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
       for (int dx = -2; dx <= 2; ++dx) {
         const int x = bx + dx;
         if (x < 0 || x >= nb[0]) continue;
+        if (sx[dx + 2] + sy[dy + 2] + sz[dz + 2] > reach2) continue;
         const int bin = b.bin_base[r] + x + nb[0] * (y + nb[1] * z);
         for (int k = bstart[bin]; k < bstart[bin + 1]; ++k) {
           const int j = items[k];
