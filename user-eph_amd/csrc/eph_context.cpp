@@ -431,7 +431,7 @@ int eph_context::resolve_timing() {
     }
     it->ms += ms;
     it->count += 1;
-    event_pool.push_back(t.a);
+    if (t.own_a) event_pool.push_back(t.a);
     event_pool.push_back(t.b);
   }
   pending.clear();
@@ -458,6 +458,25 @@ int eph_context::dev_final_integrate() {
   return EPH_OK;
 }
 
+// The density sweep; with timing on, the sweep kernel alone is "density_w",
+// the candidate check before it "cand_check" and the flag update after it
+// "cand_flags".
+int eph_context::timed_density(const AtomsView& a, const StepParams& p, const WorkView& w) {
+  if (!timing) {
+    EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
+    return EPH_OK;
+  }
+  hipEvent_t e0 = take_event(), ev[2] = {take_event(), take_event()};
+  (void)hipEventRecord(e0, stream);
+  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density, ev));
+  hipEvent_t e1 = take_event();
+  (void)hipEventRecord(e1, stream);
+  pending.push_back({"cand_check", e0, ev[0], true});
+  pending.push_back({"density_w", ev[0], ev[1], false});
+  pending.push_back({"cand_flags", ev[1], e1, false});
+  return EPH_OK;
+}
+
 int eph_context::dev_post_force() {
   if (!have_atoms || !have_list) return fail(EPH_ERR_STATE, "atoms and neighbour list must be set");
   if (atomic()) return eph::atomic_post_force(this);
@@ -476,9 +495,7 @@ int eph_context::dev_post_force() {
   // models other than PRL need rho (and xi) but no w sum
   StepParams pd = p;
   if (model != 4) pd.flags &= ~EPH_FRICTION;
-  timed_begin("density_w", &e);
-  EPH_HIP(this, launch_density_w(a, tabs, pd, w, stream, form_density));
-  timed_end("density_w", e);
+  if (int rc = timed_density(a, pd, w)) return rc;
   if (model == 4) {
     timed_begin("ghost_rec", &e);
     EPH_HIP(this, launch_ghost_rec(a, w, stream));
@@ -513,15 +530,11 @@ int eph_context::dev_post_force_local() {
   p.xi_mode = (flags & EPH_RANDOM) ? (xi_mode_next ? 1 : 2) : 0;
   const AtomsView a = atoms_view();
   const WorkView w = work_view();
-  hipEvent_t e;
   if (!pv_fresh || !a.pv) {
     EPH_HIP(this, launch_pack_pv(a, w, stream));
     pv_fresh = a.pv != nullptr;
   }
-  timed_begin("density_w", &e);
-  EPH_HIP(this, launch_density_w(a, tabs, p, w, stream, form_density));
-  timed_end("density_w", e);
-  return EPH_OK;
+  return timed_density(a, p, w);
 }
 
 // the second part, once the ghosts' records hold their exchanged values:

@@ -134,7 +134,9 @@ struct eph_context {
   bool stage_failed = false;  // no pinned memory: the runtime copies directly
   // optional per-kernel timing
   bool timing = false;
-  struct TimedLaunch { const char* name; hipEvent_t a, b; };
+  // own_a: a goes back to the pool with this entry (false when an earlier
+  // entry's b is this entry's a)
+  struct TimedLaunch { const char* name; hipEvent_t a, b; bool own_a = true; };
   std::vector<TimedLaunch> pending;
   std::vector<hipEvent_t> event_pool;
   struct KernelStat { std::string name; double ms = 0; long count = 0; };
@@ -172,6 +174,7 @@ struct eph_context {
   int d2h(void* dst, const void* src, size_t bytes);
   // timing helpers
   hipEvent_t take_event();
+  int timed_density(const eph::AtomsView& a, const eph::StepParams& p, const eph::WorkView& w);
   void timed_begin(const char* name, hipEvent_t* a);
   void timed_end(const char* name, hipEvent_t a);
   int resolve_timing();

@@ -202,8 +202,10 @@ hipError_t launch_csr_to_ell(int nlocal, const int* off, const int* idx, int nei
 // kLdsForm1024/512/256 persistent blocks of that many threads with the
 // tables in LDS (when they fit); -1 by size (LDS with 1024 threads for the
 // density sweep, 512 for the force sweep)
+// ev (optional): two events recorded around the sweep kernel alone (not the
+// candidate check before it and the flag update after it)
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
-                            const WorkView& w, hipStream_t s, int form = -1);
+                            const WorkView& w, hipStream_t s, int form = -1, hipEvent_t* ev = nullptr);
 // the tables fit in one workgroup's LDS on the current device
 bool lds_tables_possible(const TablesView& t);
 hipError_t launch_pack_pv(const AtomsView& a, const WorkView& w, hipStream_t s);

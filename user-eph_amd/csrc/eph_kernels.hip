@@ -2202,7 +2202,7 @@ static void density_lds(int blk, size_t lds, const AtomsView& a, const TablesVie
 static int lds_block(int form) { return form == kLdsForm256 ? 256 : form == kLdsForm512 ? 512 : 1024; }
 
 hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepParams& p,
-                            const WorkView& w, hipStream_t s, int form) {
+                            const WorkView& w, hipStream_t s, int form, hipEvent_t* ev) {
   if (a.nlocal == 0) return hipSuccess;
   const bool fric = p.flags & 0x01;
   const bool multi = t.n_elem > 1;
@@ -2210,6 +2210,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   if (f >= kLdsForm1024 && f <= kLdsForm256) {
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int blk = lds_block(f);
+    if (ev) (void)hipEventRecord(ev[0], s);
 #define EPH_DWL_M(XI, F) if (multi) density_lds<XI, F, true>(blk, lds, a, t, p, w, s); \
                          else density_lds<XI, F, false>(blk, lds, a, t, p, w, s)
 #define EPH_DWL_F(XI) if (fric) { EPH_DWL_M(XI, true); } else { EPH_DWL_M(XI, false); }
@@ -2220,6 +2221,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
     }
 #undef EPH_DWL_F
 #undef EPH_DWL_M
+    if (ev) (void)hipEventRecord(ev[1], s);
     return hipGetLastError();
   }
   const int g = nblk(a.nlocal);
@@ -2227,6 +2229,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
   WorkView wc = w;
   if (!keep) wc.rcand2 = 0.0;
   if (keep) k_cand_check<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal + a.nghost, a.pos, wc);
+  if (ev) (void)hipEventRecord(ev[0], s);  // the sweep alone, without the candidate check
   const size_t dl = chunk_lds(f, kDensityStageBytes);
 #define EPH_DW(XI, F, M) k_density_w<XI, F, M><<<g, kBlock, dl, s>>>(a, t, p, wc)
 #define EPH_DW_M(XI, F) if (multi) EPH_DW(XI, F, true); else EPH_DW(XI, F, false)
@@ -2239,6 +2242,7 @@ hipError_t launch_density_w(const AtomsView& a, const TablesView& t, const StepP
 #undef EPH_DW_F
 #undef EPH_DW_M
 #undef EPH_DW
+  if (ev) (void)hipEventRecord(ev[1], s);
   if (keep) k_cand_flags<<<1, 64, 0, s>>>(w.cstate, w.moved_cap);
   return hipGetLastError();
 }
