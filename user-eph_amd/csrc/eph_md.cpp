@@ -555,7 +555,9 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     const long long step = first_step + st;
     ctx->step = step;
     hipEvent_t* ev = evs.data() + 5 * static_cast<size_t>(st);
-    EPH_HIP(md, hipEventRecord(ev[0], s));
+    // a step starts where the previous one ended: the previous step's end
+    // event serves as this step's start (one event less between two kernels)
+    if (st == 0) EPH_HIP(md, hipEventRecord(ev[0], s));
     if ((rc = ctx->dev_initial_integrate())) break;
     EPH_HIP(md, hipEventRecord(ev[1], s));
     bool host_rebuild = false;
@@ -623,11 +625,12 @@ int eph_md_run(eph_md* md, eph_context* ctx, int nsteps, long long first_step, d
     EPH_HIP(md, hipEventSynchronize(evs[5 * static_cast<size_t>(nsteps) - 1]));
     for (int st = 0; st < nsteps; ++st) {
       const hipEvent_t* ev = evs.data() + 5 * static_cast<size_t>(st);
+      const hipEvent_t start = st == 0 ? ev[0] : ev[-1];
       float a = 0, b = 0, c = 0, d = 0, e = 0;
-      EPH_HIP(md, hipEventElapsedTime(&a, ev[0], ev[1]));
+      EPH_HIP(md, hipEventElapsedTime(&a, start, ev[1]));
       EPH_HIP(md, hipEventElapsedTime(&b, ev[3], ev[4]));
       EPH_HIP(md, hipEventElapsedTime(&c, ev[2], ev[3]));
-      EPH_HIP(md, hipEventElapsedTime(&d, ev[0], ev[4]));
+      EPH_HIP(md, hipEventElapsedTime(&d, start, ev[4]));
       if (!host_rebuilt[st]) EPH_HIP(md, hipEventElapsedTime(&e, ev[1], ev[2]));
       t_fix += a + b;
       t_pair += c;
