@@ -77,7 +77,7 @@ static __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-constexpr int kCoopStage = 576;  // 16-B units of LDS per wave and force-sweep entry
+constexpr int kCoopStage = 384;  // 16-B units of LDS per wave and force-sweep entry (64 x 96 B)
 constexpr int kCoopF = 1;        // force-sweep entries per iteration
 // density sweep: entries per iteration of the rho loop (128 x 16 B of LDS
 // each) and of the w loop (256 x 16 B each); their copies need no padding,
@@ -740,38 +740,27 @@ __device__ __forceinline__ void prl_forces_atom(const int i, const AtomsView& a,
 }
 
 // Cooperative form: the wave fetches its 64 atoms' k-th neighbour records
-// together, 8 lanes per 128-B record, straight into LDS (global_load_lds, 16 B
-// per lane), and each lane then reads its own record from there.  A gather
-// instruction thus touches 8 cache lines instead of 64.  In copy m, lane
-// l = 8q + s fetches slot s of the record of lane 8m + q, which holds part
-// (s - q) mod 8 (parts 0-5: p, rw, xa; 6-7: padding, not fetched); the
-// rotation makes the per-lane reads free of bank conflicts.  Copy m lands at
-// stage + 72 m (16-B units: 1 KB plus 128 B of padding, so that lanes L and
-// L + 8 fall in different bank halves).  The records of entry k + 1 are in
-// flight while entry k is computed.  Every lane of the wave runs the loop to
-// the wave's longest active list (lanes past their own do only the copies),
-// so the whole wave must call this; `valid` = lane i is a local atom.
-// stage: 576 x 16 B of LDS owned by the calling wave.
-
-static __device__ __forceinline__ void coop_fetch(const double2* __restrict__ recs, int j, int q8,
-                                                  int part, double2* stage) {
-  // the record indices of lanes q8, 8 + q8, ..., 56 + q8, all read before the
-  // first copy is issued
-  int r0 = __shfl(j, q8), r1 = __shfl(j, 8 + q8), r2 = __shfl(j, 16 + q8), r3 = __shfl(j, 24 + q8);
-  int r4 = __shfl(j, 32 + q8), r5 = __shfl(j, 40 + q8), r6 = __shfl(j, 48 + q8), r7 = __shfl(j, 56 + q8);
-  asm volatile("" : "+v"(r0), "+v"(r1), "+v"(r2), "+v"(r3), "+v"(r4), "+v"(r5), "+v"(r6), "+v"(r7));
-  if (part < 6) {
-    using lds_ptr = __attribute__((address_space(3))) void*;
-    const double2* g = recs + part;
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r0) * 8, (lds_ptr)(stage + 0), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r1) * 8, (lds_ptr)(stage + 72), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r2) * 8, (lds_ptr)(stage + 144), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r3) * 8, (lds_ptr)(stage + 216), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r4) * 8, (lds_ptr)(stage + 288), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r5) * 8, (lds_ptr)(stage + 360), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r6) * 8, (lds_ptr)(stage + 432), 16, 0, 0);
-    __builtin_amdgcn_global_load_lds(g + static_cast<size_t>(r7) * 8, (lds_ptr)(stage + 504), 16, 0, 0);
-  }
+// together, straight into LDS (global_load_lds, 16 B per lane), and each lane
+// then reads its own record from there.  Only parts 0-5 of each 128-B record
+// (p, (w, rho), (xi, alpha): 96 B) are copied, packed: 6 copies of 64 lanes
+// for the wave's 64 records, every lane busy.  In copy m lane l takes slot
+// g = 64 m + l of the stage, which receives part g mod 6 of the record of
+// lane g / 6, so record r lands at stage + 6 r (16-B units).  A gather
+// instruction touches at most 11 cache lines instead of 64.  Every lane of the
+// wave runs the loop to the wave's longest active list (lanes past their own
+// do only the copies), so the whole wave must call this; `valid` = lane i is
+// a local atom.  stage: kCoopStage x 16 B of LDS owned by the calling wave.
+static __device__ __forceinline__ void coop_fetch6(const double2* __restrict__ recs, int j, int lane,
+                                                   double2* stage) {
+  using lds_ptr = __attribute__((address_space(3))) void*;
+  int r[6];  // the record indices, all read before the first copy is issued
+#pragma unroll
+  for (int m = 0; m < 6; ++m) r[m] = __shfl(j, (64 * m + lane) / 6);
+  asm volatile("" : "+v"(r[0]), "+v"(r[1]), "+v"(r[2]), "+v"(r[3]), "+v"(r[4]), "+v"(r[5]));
+#pragma unroll
+  for (int m = 0; m < 6; ++m)
+    __builtin_amdgcn_global_load_lds(recs + static_cast<size_t>(r[m]) * 8 + (64 * m + lane) % 6,
+                                     (lds_ptr)(stage + 64 * m), 16, 0, 0);
 }
 
 template <bool FRIC, bool RAND, bool MULTI>
@@ -794,9 +783,7 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
   for (int o = 32; o > 0; o >>= 1) nmax = max(nmax, __shfl_xor(nmax, o));
   double fe[3] = {0.0, 0.0, 0.0}, fr[3] = {0.0, 0.0, 0.0};
   const double2* __restrict__ recs = reinterpret_cast<const double2*>(w.rec);
-  const int q8 = lane >> 3, part = ((lane & 7) - q8) & 7;
-  const int rot = lane & 7;
-  const double2* mine = stage + 72 * (lane >> 3) + 8 * (lane & 7);
+  const double2* mine6 = stage + 6 * lane;
   static_assert(kCoopF == 1, "one entry per iteration");
   // Two-stage pipeline.  Iteration k waits for everything in flight (the
   // coefficients of entry k, the records of entry k + 1), reads entry k + 1's
@@ -816,7 +803,7 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
   auto read_entry = [&](double4& pj, double4& rwj, double4& xaj) {
     double2 sv[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) sv[c] = mine[(c + rot) & 7];
+    for (int c = 0; c < 6; ++c) sv[c] = mine6[c];
     __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
     pj = make_double4(sv[0].x, sv[0].y, sv[1].x, sv[1].y);
     rwj = make_double4(sv[2].x, sv[2].y, sv[3].x, sv[3].y);
@@ -859,7 +846,7 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     wave_lds_sync();
     take(nxt, k + 1);
-    coop_fetch(recs, k + 2 < na ? jraw : 0, q8, part, stage);
+    coop_fetch6(recs, k + 2 < na ? jraw : 0, lane, stage);
     jraw = w.act[row(k + 3) * stride + i];
     // the loads above go out first; pair k's arithmetic (whose inputs are
     // all in registers, its loads having completed at the top) comes after
@@ -871,12 +858,12 @@ __device__ __forceinline__ void prl_forces_atom_coop(const int i_lane, const boo
   };
   Entry e0, e1;
   if (nmax > 0) {
-    coop_fetch(recs, na > 0 ? w.act[i] : 0, q8, part, stage);
+    coop_fetch6(recs, na > 0 ? w.act[i] : 0, lane, stage);
     jraw = w.act[row(1) * stride + i];
     __builtin_amdgcn_s_waitcnt(kWaitVm0);
     wave_lds_sync();
     take(e0, 0);
-    coop_fetch(recs, 1 < na ? jraw : 0, q8, part, stage);
+    coop_fetch6(recs, 1 < na ? jraw : 0, lane, stage);
     jraw = w.act[row(2) * stride + i];
   }
   // two entries per trip, the two Entry slots taking turns (no copies of
