@@ -100,3 +100,62 @@ def test_ghosts_moving_with_the_box_keep_bits(oracle, tmp_path):
     # ghosts' displacements were seen
     assert grew, "the shrinking box brought no new active pair"
     assert moved
+
+
+def _step_to(p, step, x_local):
+    """One Verlet step of both fixes with the local positions set to x_local
+    after the first half-kick (ghost images follow their owners)."""
+    A, B = p.sA, p.sB
+    n = A.nlocal
+    p.ora.initial_integrate(A)
+    p.gpu.initial_integrate(B)
+    for s in (A, B):
+        s.x[:n] = x_local
+        s.x[n:] = s.x[s.ghost_owner] + s.ghost_shift
+        s.v[n:] = s.v[s.ghost_owner]
+        s.f[:n] = 0.0
+    p.ora.post_force(A, step=step)
+    p.gpu.post_force(B, step=step)
+    p.ora.final_integrate(A)
+    p.gpu.final_integrate(B)
+    p.ora.end_of_step(A, step)
+    p.gpu.end_of_step(B, step)
+
+
+def test_overflowing_moved_list_then_return_keeps_bits(oracle, tmp_path):
+    """More than 512 atoms move at once (the moved list overflows), then all
+    return to the lattice.  Atoms i and j, third-shell neighbours (4.43 A,
+    inside r_c = 4.5 A), are pulled apart to 5.03 A (outside r_c + delta =
+    4.95 A) on the overflowing step and come back on the next.  Back on the
+    lattice no atom is 0.45 delta from the positions the candidates were last
+    checked against, yet i's candidates must again hold j: the overflowing
+    step may not leave candidates taken at its own positions behind.  Bit for
+    bit against the oracle on the steps after the return."""
+    print("candidates: fcc10, moved-list overflow and return, 4 steps", flush=True)
+    p = Pair(oracle, ea, str(tmp_path), cells=10, T=0.0, grid=5, dt=2e-4)
+    A = p.sA
+    n = A.nlocal
+    x0 = A.x[:n].copy()
+    c = 0.5 * (x0.min(0) + x0.max(0))
+    i = int(np.argmin(((x0 - c) ** 2).sum(1)))
+    d = np.sqrt(((x0 - x0[i]) ** 2).sum(1))
+    j = int(np.argmin(np.abs(d - 3.615 * np.sqrt(1.5))))
+    assert abs(d[j] - 3.615 * np.sqrt(1.5)) < 1e-9
+    u = (x0[j] - x0[i]) / d[j]
+    x1 = x0.copy()
+    x1[i] -= 0.3 * u
+    x1[j] += 0.3 * u
+    others = np.setdiff1d(np.arange(n), [i, j])[-600:]
+    x1[others, 2] += 0.25
+    assert np.sqrt(((x1[j] - x1[i]) ** 2).sum()) > 4.95 + 0.05
+    _step_to(p, 1, x0)
+    _step_to(p, 2, x1)
+    st = p.gpu.candidate_stats()
+    assert st["moved"] > 512 and st["full_waves"] == st["waves"], st
+    _step_to(p, 3, x0)
+    _same(p, "step 3 (back on the lattice)")
+    assert p.gpu.candidate_stats()["retaken"]
+    _step_to(p, 4, x0)
+    _same(p, "step 4")
+    st = p.gpu.candidate_stats()
+    assert not st["retaken"] and st["full_waves"] == 0, st

@@ -156,6 +156,7 @@ int eph_ensemble::setup(long long first) {
   const int st = sys->stride;
   EPH_HIP(this, sys->nbr.reserve((size_t)cap_f * st));
   EPH_HIP(this, sys->act.reserve((size_t)cap_f * st));
+  EPH_HIP(this, sys->cand.reserve((size_t)cap_f * st));  // the density sweep's candidate lists
   EPH_HIP(this, sys->ghost_owner.reserve(ghost_cap));
   EPH_HIP(this, sys->ghost_shift.reserve(ghost_cap));
   EPH_HIP(this, lj_nbr.reserve((size_t)cap_l * st));
@@ -207,6 +208,17 @@ int eph_ensemble::setup(long long first) {
   w.go = flags.p; w.overflow = flags.p + 1; w.nrebuild = flags.p + 2; w.nghost = flags.p + 4;
   w.cnt = cnt.p; w.start = start.p; w.bin_of = bin_of.p; w.bcount = bcount.p; w.bstart = bstart.p;
   w.bfill = bfill.p; w.items = items.p; w.scan_tmp = scan_tmp.p; w.xref = xref.p;
+  // each rebuild also writes the density sweep's candidate lists, valid at
+  // the rebuild's positions (the buffers keep their size for the ensemble's life)
+  {
+    sys->drop_candidates();
+    const WorkView cw = sys->work_view();
+    w.lists.cand = cw.rcand2 > 0.0 ? cw.cand : nullptr;
+    w.lists.ncand = cw.ncand;
+    w.lists.cand2 = cw.rcand2;
+    w.cref = cw.cref;
+    w.cand_valid = sys->cstate.p;
+  }
   // initial ghosts and lists: a forced device build
   EPH_HIP(this, launch_fill_int(flags.p, 8, 0, s));
   EPH_HIP(this, launch_nb_force(w.go, s));
@@ -320,7 +332,7 @@ int eph_ensemble::enqueue_step(int nlog) {
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
   EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
   EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s,
-                                       var_density));
+                                       var_density, ng));
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   if (sys->model == 4)
     EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, p, w, chunks.p, nchunk, s,
@@ -398,7 +410,7 @@ int eph_ensemble::autotune(double* ms) {
   StepParams p = sys->step_params();
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
   EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
-  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s, 0));
+  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s, 0, ng));
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   StepParams pf = p;
   pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
@@ -418,7 +430,7 @@ int eph_ensemble::autotune(double* ms) {
         he = hipEventRecord(e0, s);
         if (he == hipSuccess)
           he = phase == 0 ? launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
-                                                   step_dev.p, s, v)
+                                                   step_dev.p, s, v, ng)
                           : launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p, nchunk,
                                                     s, v);
         if (he == hipSuccess) he = hipEventRecord(e1, s);

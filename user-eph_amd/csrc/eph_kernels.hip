@@ -313,15 +313,16 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   const int lane = threadIdx.x & 63;
   const double4 pi = a.pos[i];
   const bool grp = valid && (a.mask[i] & a.groupbit) != 0;
-  // the rho loop walks the candidate list when it is valid and no moved atom
-  // is near this wave; it scans the full list otherwise, and takes the
-  // candidates on the way when they are being retaken for all atoms
+  // the rho loop walks the candidate list when it is valid, the moved list
+  // holds every moved atom and none is near this wave; it scans the full
+  // list otherwise, and takes the candidates on the way when they are being
+  // retaken for all atoms (the reference positions are the current ones)
   const bool keep = w.rcand2 > 0.0;
   bool use_cand = false, take = false;
   if (keep) {
     const int nm = w.cstate[2];
-    take = w.cstate[0] == 0 || nm > w.moved_cap;
-    use_cand = !take && !(nm > 0 && wave_near_moved(valid, pi, w, nm));
+    take = w.cstate[0] == 0;
+    use_cand = !take && nm <= w.moved_cap && !(nm > 0 && wave_near_moved(valid, pi, w, nm));
     if (!use_cand && lane == 0) atomicAdd(w.cstate + 1, 1);
   }
   const int* __restrict__ list = use_cand ? w.cand : w.nbr;
@@ -444,15 +445,39 @@ __global__ __launch_bounds__(kBlock) void k_cand_check(int nall, const double4* 
   }
 }
 
-// After the sweep: a sweep that retook the candidates leaves them valid; the
-// moved list starts empty at the next check
+// the same check with the ghost count read on the device (the ensemble's
+// recorded steps, whose ghost count changes between replays); the grid covers
+// nlocal + the ghost capacity
+__global__ __launch_bounds__(kBlock) void k_cand_check_dev(int nlocal, const int* __restrict__ ng,
+                                                           const double4* __restrict__ pos, WorkView w) {
+  const int i = blockIdx.x * kBlock + threadIdx.x;
+  const int nall = nlocal + *ng;
+  if (w.cstate[0] == 0) {
+    if (i < nall) w.cref[i] = pos[i];
+    return;
+  }
+  if (i >= nall) return;
+  const double4 x = pos[i], r = w.cref[i];
+  const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
+  if (!(dx * dx + dy * dy + dz * dz < w.half_delta2)) {
+    const int k = atomicAdd(w.cstate + 2, 1);
+    if (k < w.moved_cap) w.moved[k] = x;
+  }
+}
+
+// After the sweep: a sweep that retook the candidates leaves them valid.  A
+// moved list that overflowed sent every wave to its full list; the reference
+// positions then no longer bound how far the atoms are from where they were
+// when the candidates were taken, so the next check takes new reference
+// positions and the next sweep retakes the candidates.  The moved list starts
+// empty at the next check.
 __global__ void k_cand_flags(int* cstate, int moved_cap) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  const int retake = cstate[0] == 0 || cstate[2] > moved_cap;
+  const int retake = cstate[0] == 0;
   cstate[3] = cstate[1];
   cstate[4] = cstate[2];
   cstate[5] = retake;
-  if (retake) cstate[0] = 1;
+  cstate[0] = retake || cstate[2] <= moved_cap ? 1 : 0;
   cstate[1] = 0;
   cstate[2] = 0;
 }
@@ -2547,13 +2572,13 @@ static int batch_lds_block(int form) {
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w_in, const Chunk* chunks, int nchunks,
                                   const uint32_t* seeds, const uint64_t* step, hipStream_t s,
-                                  int variant) {
+                                  int variant, const int* ng) {
   WorkView w = w_in;
-  w.rcand2 = 0.0;  // replicas walk their full lists
   if (nchunks == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
   const int bs = batch_lds_block(variant);
   if (bs > 0 && batch_lds_possible(t)) {
+    w.rcand2 = 0.0;  // the LDS-table forms walk the full lists
     const size_t lds = static_cast<size_t>(t.n_elem) * t.n_rho * sizeof(double4);
     const int ntiles = (nchunks + bs / kBlock - 1) / (bs / kBlock);
 #define EPH_DWBL(XI, F, M, BS)                                                                   \
@@ -2576,6 +2601,13 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
 #undef EPH_DWBL
     return hipGetLastError();
   }
+  // candidate lists over all replicas' atoms (moved atoms are matched by
+  // position, so a replica's moved atom may also send waves of another
+  // replica overlapping it in space to their full lists: more work, never a
+  // missed pair); they need the device ghost count
+  const bool keep = w.rcand2 > 0.0 && w.cstate && ng;
+  if (!keep) w.rcand2 = 0.0;
+  if (keep) k_cand_check_dev<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal, ng, a.pos, w);
   const size_t dl = chunk_lds(variant, kDensityStageBytes);
 #define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, dl, s>>>(a, t, p, w, chunks, seeds, step)
 #define EPH_DWB_M(XI, F) if (multi) EPH_DWB(XI, F, true); else EPH_DWB(XI, F, false)
@@ -2588,6 +2620,7 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
 #undef EPH_DWB_F
 #undef EPH_DWB_M
 #undef EPH_DWB
+  if (keep) k_cand_flags<<<1, 64, 0, s>>>(w.cstate, w.moved_cap);
   return hipGetLastError();
 }
 
