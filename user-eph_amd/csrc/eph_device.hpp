@@ -279,11 +279,13 @@ hipError_t launch_copy_probe(double4* dst, const double4* src, long n, hipStream
 // to 0 when the tables do not fit).  Both give the same bits.
 bool batch_lds_possible(const TablesView& t);
 // ng: the device ghost count (the ghost slots of a hold a.nghost = capacity);
-// with it the cache-table forms walk candidate lists, without it full lists
+// rep: the replica of each of those atoms.  With both the cache-table forms
+// walk candidate lists, without them full lists.
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w, const Chunk* chunks, int nchunks,
                                   const uint32_t* seeds, const uint64_t* step, hipStream_t s,
-                                  int variant = 0, const int* ng = nullptr);
+                                  int variant = 0, const int* ng = nullptr,
+                                  const int* rep = nullptr);
 hipError_t launch_prl_forces_batch(const AtomsView& a, const TablesView& t, const GridView* grids,
                                    const StepParams& p, const WorkView& w, const Chunk* chunks,
                                    int nchunks, hipStream_t s, int variant = 0);

@@ -332,7 +332,7 @@ int eph_ensemble::enqueue_step(int nlog) {
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
   EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
   EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s,
-                                       var_density, ng));
+                                       var_density, ng, rep.p));
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   if (sys->model == 4)
     EPH_HIP(this, launch_prl_forces_batch(a, sys->tabs, gviews.p, p, w, chunks.p, nchunk, s,
@@ -410,7 +410,7 @@ int eph_ensemble::autotune(double* ms) {
   StepParams p = sys->step_params();
   p.xi_mode = (sys->flags & EPH_RANDOM) ? 2 : 0;
   EPH_HIP(this, launch_pack_pv_dev(a, w, ng, ghost_cap, s));
-  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s, 0, ng));
+  EPH_HIP(this, launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p, step_dev.p, s, 0, ng, rep.p));
   EPH_HIP(this, launch_ghost_rec_dev(a, w, ng, ghost_cap, s));
   StepParams pf = p;
   pf.flags |= EPH_NOFRICTION | EPH_NORANDOM;
@@ -426,18 +426,18 @@ int eph_ensemble::autotune(double* ms) {
     for (int v = 0; v < nf && he == hipSuccess; ++v) {
       if (v >= kLdsForm1024 && v <= kLdsForm256 && !lds_ok) continue;
       double acc = 0.0;
-      for (int rep = 0; rep < 4 && he == hipSuccess; ++rep) {  // first launch warms up
+      for (int trial = 0; trial < 4 && he == hipSuccess; ++trial) {  // first launch warms up
         he = hipEventRecord(e0, s);
         if (he == hipSuccess)
           he = phase == 0 ? launch_density_w_batch(a, sys->tabs, p, w, chunks.p, nchunk, seeds.p,
-                                                   step_dev.p, s, v, ng)
+                                                   step_dev.p, s, v, ng, rep.p)
                           : launch_prl_forces_batch(a, sys->tabs, gviews.p, pf, w, chunks.p, nchunk,
                                                     s, v);
         if (he == hipSuccess) he = hipEventRecord(e1, s);
         if (he == hipSuccess) he = hipEventSynchronize(e1);
         float m = 0;
         if (he == hipSuccess) he = hipEventElapsedTime(&m, e0, e1);
-        if (rep > 0) acc += m / 3.0;
+        if (trial > 0) acc += m / 3.0;
       }
       t[phase * nf + v] = acc;
     }

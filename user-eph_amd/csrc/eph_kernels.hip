@@ -272,9 +272,12 @@ __device__ __forceinline__ void density_w_atom(const int i, const AtomsView& a, 
 
 // Is an atom listed as moved within w.reach of any of this wave's atoms?  The
 // moved positions are tested against the wave's bounding box grown by reach,
-// a superset of the spheres around its atoms; lanes share the list.
+// a superset of the spheres around its atoms; lanes share the list.  With
+// rep >= 0 (a batch of replicas, whose lists never pair atoms of two
+// replicas) only the moved atoms of replica rep count: the check writes each
+// moved atom's replica into .w.
 __device__ __forceinline__ bool wave_near_moved(const bool valid, const double4& pi,
-                                                const WorkView& w, const int nm) {
+                                                const WorkView& w, const int nm, const int rep) {
   const double inf = __builtin_huge_val();
   double lx = valid ? pi.x : inf, ly = valid ? pi.y : inf, lz = valid ? pi.z : inf;
   double hx = valid ? pi.x : -inf, hy = valid ? pi.y : -inf, hz = valid ? pi.z : -inf;
@@ -292,7 +295,8 @@ __device__ __forceinline__ bool wave_near_moved(const bool valid, const double4&
     const int m = b + lane;
     if (m < n) {
       const double4 q = w.moved[m];
-      hit = hit || (q.x >= lx && q.x <= hx && q.y >= ly && q.y <= hy && q.z >= lz && q.z <= hz);
+      hit = hit || ((rep < 0 || q.w == static_cast<double>(rep)) && q.x >= lx && q.x <= hx &&
+                    q.y >= ly && q.y <= hy && q.z >= lz && q.z <= hz);
     }
   }
   return __any(hit) != 0;
@@ -302,12 +306,13 @@ __device__ __forceinline__ bool wave_near_moved(const bool valid, const double4&
 // positions of the rho loop and the (position, velocity) records of the w loop
 // come through LDS, see coop_fetch_pos / coop_fetch_pv.  The records of entry
 // k + 1 are in flight while entry k is computed.  `valid` = lane i is a local
-// atom; stage: kCoopStageW x 16 B of LDS owned by the calling wave.
+// atom; stage: kCoopStageW x 16 B of LDS owned by the calling wave; rep: the
+// wave's replica in a batch, -1 otherwise.
 template <int XI, bool FRIC, bool MULTI>
 __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool valid, const AtomsView& a,
                                                     const TablesView& t, const double4* __restrict__ rtab,
                                                     const StepParams& p, const WorkView& w,
-                                                    double2* stage) {
+                                                    double2* stage, const int rep = -1) {
   const int i = valid ? i_lane : 0;  // lanes past the end read atom 0 and store nothing
   const int stride = a.stride;
   const int lane = threadIdx.x & 63;
@@ -322,7 +327,7 @@ __device__ __forceinline__ void density_w_atom_coop(const int i_lane, const bool
   if (keep) {
     const int nm = w.cstate[2];
     take = w.cstate[0] == 0;
-    use_cand = !take && nm <= w.moved_cap && !(nm > 0 && wave_near_moved(valid, pi, w, nm));
+    use_cand = !take && nm <= w.moved_cap && !(nm > 0 && wave_near_moved(valid, pi, w, nm, rep));
     if (!use_cand && lane == 0) atomicAdd(w.cstate + 1, 1);
   }
   const int* __restrict__ list = use_cand ? w.cand : w.nbr;
@@ -445,11 +450,13 @@ __global__ __launch_bounds__(kBlock) void k_cand_check(int nall, const double4* 
   }
 }
 
-// the same check with the ghost count read on the device (the ensemble's
-// recorded steps, whose ghost count changes between replays); the grid covers
-// nlocal + the ghost capacity
+// the same check for a batch of replicas, with the ghost count read on the
+// device (the ensemble's recorded steps, whose ghost count changes between
+// replays; the grid covers nlocal + the ghost capacity); a moved atom's entry
+// carries its replica rep[i] in .w
 __global__ __launch_bounds__(kBlock) void k_cand_check_dev(int nlocal, const int* __restrict__ ng,
-                                                           const double4* __restrict__ pos, WorkView w) {
+                                                           const double4* __restrict__ pos,
+                                                           const int* __restrict__ rep, WorkView w) {
   const int i = blockIdx.x * kBlock + threadIdx.x;
   const int nall = nlocal + *ng;
   if (w.cstate[0] == 0) {
@@ -461,7 +468,7 @@ __global__ __launch_bounds__(kBlock) void k_cand_check_dev(int nlocal, const int
   const double dx = x.x - r.x, dy = x.y - r.y, dz = x.z - r.z;
   if (!(dx * dx + dy * dy + dz * dz < w.half_delta2)) {
     const int k = atomicAdd(w.cstate + 2, 1);
-    if (k < w.moved_cap) w.moved[k] = x;
+    if (k < w.moved_cap) w.moved[k] = make_double4(x.x, x.y, x.z, static_cast<double>(rep[i]));
   }
 }
 
@@ -1822,7 +1829,8 @@ __global__ __launch_bounds__(kBlock) void k_density_w_batch(AtomsView a, TablesV
   pr.seed = seeds[c.rep];
   pr.step = *step;
   density_w_atom_coop<XI, FRIC, MULTI>(c.start + threadIdx.x, static_cast<int>(threadIdx.x) < c.count,
-                                       a, t, t.rho_r2, pr, w, s_stage + (threadIdx.x >> 6) * kCoopStageW);
+                                       a, t, t.rho_r2, pr, w, s_stage + (threadIdx.x >> 6) * kCoopStageW,
+                                       c.rep);
 }
 
 // start of a recorded step: step += 1
@@ -2572,7 +2580,7 @@ static int batch_lds_block(int form) {
 hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const StepParams& p,
                                   const WorkView& w_in, const Chunk* chunks, int nchunks,
                                   const uint32_t* seeds, const uint64_t* step, hipStream_t s,
-                                  int variant, const int* ng) {
+                                  int variant, const int* ng, const int* rep) {
   WorkView w = w_in;
   if (nchunks == 0) return hipSuccess;
   const bool fric = p.flags & 0x01, multi = t.n_elem > 1;
@@ -2601,13 +2609,12 @@ hipError_t launch_density_w_batch(const AtomsView& a, const TablesView& t, const
 #undef EPH_DWBL
     return hipGetLastError();
   }
-  // candidate lists over all replicas' atoms (moved atoms are matched by
-  // position, so a replica's moved atom may also send waves of another
-  // replica overlapping it in space to their full lists: more work, never a
-  // missed pair); they need the device ghost count
-  const bool keep = w.rcand2 > 0.0 && w.cstate && ng;
+  // candidate lists over all replicas' atoms, one moved list for all (a wave
+  // looks only at its own replica's moved atoms); they need the device ghost
+  // count and the replica of every atom
+  const bool keep = w.rcand2 > 0.0 && w.cstate && ng && rep;
   if (!keep) w.rcand2 = 0.0;
-  if (keep) k_cand_check_dev<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal, ng, a.pos, w);
+  if (keep) k_cand_check_dev<<<nblk(a.nlocal + a.nghost), kBlock, 0, s>>>(a.nlocal, ng, a.pos, rep, w);
   const size_t dl = chunk_lds(variant, kDensityStageBytes);
 #define EPH_DWB(XI, F, M) k_density_w_batch<XI, F, M><<<nchunks, kBlock, dl, s>>>(a, t, p, w, chunks, seeds, step)
 #define EPH_DWB_M(XI, F) if (multi) EPH_DWB(XI, F, true); else EPH_DWB(XI, F, false)
