@@ -79,7 +79,12 @@ int eph_ensemble_size(const eph_ensemble* e, int* nrep, long long* natoms, long 
 /* replica r: local atoms, T_e cells, FDM sub-steps per MD step */
 int eph_ensemble_info(const eph_ensemble* e, int r, int* nlocal, int* ncells, int* substeps);
 
-/* which: 0 x, 1 v, 2 f ([nlocal][3]); 3 T_e ([ncells]); 4 (E_local, Ee) */
+/* which: 0 x, 1 v, 2 f ([nlocal][3]); 3 T_e ([ncells]); 4 (E_local, Ee);
+ * 5 the candidate lists at the last density sweep of the whole batch (r only
+ * has to be valid): waves that scanned their full lists, waves in all, moved
+ * atoms of all replicas, retaken (0 or 1).  Meaningful after a chunk-form
+ * sweep; the LDS-table forms always walk the full lists and leave them as
+ * they were. */
 int eph_ensemble_get(eph_ensemble* e, int r, int which, double* out);
 
 #ifdef __cplusplus

@@ -210,3 +210,38 @@ def test_ensemble_temperature_dependent_grids_match_separate_runs(tmp_path, grap
         assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), f"replica {r}: T_e"
         E = ens.get(r, "E")
         assert E[1] == fix.compute_vector(0), f"replica {r}: Ee"
+
+
+def test_ensemble_candidates_follow_each_replicas_own_moved_atoms(tmp_path):
+    """A 2 keV primary atom (0.078 A per 0.1 fs step) in a 4^3 replica (4
+    waves) next to two 8^3 replicas at rest (32 waves each) that overlap it in
+    space, it and its 26 ghost images.  From its third step it is a moved atom
+    of replica 0: waves of replica 0 near it scan their full lists while every
+    wave of the other replicas keeps walking its candidates (their lists never
+    pair with replica 0's atoms).  Each replica bit for bit equal to its
+    separate run."""
+    species, dt, nsteps = 2, 1e-4, 12
+    specs = [dict(cells=4, seed=31, fix_seed=301, pka=2000.0),
+             dict(cells=8, seed=32, fix_seed=302, pka=0.0),
+             dict(cells=8, seed=33, fix_seed=303, pka=0.0)]
+    reps = _replicas(str(tmp_path), specs, species, dt)
+    ens = _ensemble(reps, species, dt)
+    ens.set_variants("chunk", "chunk")  # the forms that walk candidate lists
+    waves0 = (ens.info(0)["nlocal"] + 63) // 64
+    rebuilt = []
+    seen = 0
+    for k in range(nsteps):
+        rebuilt.append(int(ens.run(1, graph=True)["rebuilt"][0]))
+        st = ens.candidate_stats()
+        assert st["waves"] == sum((ens.info(r)["nlocal"] + 63) // 64 for r in range(3)), st
+        if st["moved"] > 0 and not st["retaken"]:
+            seen += 1
+            assert 0 < st["full_waves"] <= waves0, (k + 1, st)
+    assert seen >= 5, f"the primary atom was a moved atom on only {seen} steps"
+    for r, (kw, args) in enumerate(reps):
+        md, fix = _separate(kw, args, species, dt, np.array(rebuilt))
+        n = ens.info(r)["nlocal"]
+        xs, vs, _ = fix.dev_atoms()
+        assert np.array_equal(ens.get(r, "x"), xs[:n]), f"replica {r}: x"
+        assert np.array_equal(ens.get(r, "v"), vs[:n]), f"replica {r}: v"
+        assert np.array_equal(ens.get(r, "T"), fix.grid().get("T")), f"replica {r}: T_e"

@@ -699,6 +699,15 @@ int eph_ensemble_get(eph_ensemble* e, int r, int which, double* out) {
       EPH_HIP(e, hipMemcpyAsync(out, e->E_local.p + r, sizeof(double), hipMemcpyDeviceToHost, s));
       EPH_HIP(e, hipMemcpyAsync(out + 1, e->Ee.p + r, sizeof(double), hipMemcpyDeviceToHost, s));
       break;
+    case 5: {  // candidate-list statistics of the last batch density sweep
+      int st[8] = {};
+      if (c->cstate.p) EPH_HIP(e, hipMemcpyAsync(st, c->cstate.p, sizeof st, hipMemcpyDeviceToHost, s));
+      EPH_HIP(e, hipStreamSynchronize(s));
+      long long waves = 0;  // chunks are whole blocks of one replica, so waves per replica
+      for (const auto& q : e->reps) waves += (q.nlocal + 63) / 64;
+      out[0] = st[3]; out[1] = static_cast<double>(waves); out[2] = st[4]; out[3] = st[5];
+      return EPH_OK;
+    }
     default:
       return e->fail(EPH_ERR_ARGS, "unknown quantity");
   }

@@ -751,6 +751,14 @@ class Ensemble:
         self._check(lib().eph_ensemble_get(self.h, r, k, _d(out)))
         return out
 
+    def candidate_stats(self):
+        """The batch density sweep's candidate lists at its last chunk-form run:
+        waves that scanned the full list, waves in all, moved atoms, retaken"""
+        out = np.zeros(4)
+        self._check(lib().eph_ensemble_get(self.h, 0, 5, _d(out)))
+        return {"full_waves": int(out[0]), "waves": int(out[1]), "moved": int(out[2]),
+                "retaken": bool(out[3])}
+
 
 def spline_jacobian(dx, y):
     """d(table coefficients)/d(knot values) of spline_build: shape (n, 4, n)."""
