@@ -203,6 +203,7 @@ int eph_ensemble::setup(long long first) {
   w.box.lo = box_d.p; w.box.hi = box_d.p + 3 * R; w.box.L = box_d.p + 6 * R; w.box.glo = box_d.p + 9 * R;
   w.box.nb = nbi.p; w.box.bin_base = nbi.p + 3 * R; w.box.bs = bs; w.box.cg = cmax;
   w.lists.nbr_f = sys->nbr.p; w.lists.nn_f = sys->nnbr.p; w.lists.cap_f = cap_f; w.lists.cf2 = cf * cf;
+  w.lists.sort_scale = list_sort_scale(w.lists.cf2);
   w.lists.nbr_l = lj_nbr.p; w.lists.nn_l = lj_nn.p; w.lists.cap_l = cap_l; w.lists.cl2 = cl * cl;
   w.lists.stride = st;
   w.go = flags.p; w.overflow = flags.p + 1; w.nrebuild = flags.p + 2; w.nghost = flags.p + 4;

@@ -86,8 +86,9 @@ void eph_md::build_ghosts() {
   }
 }
 
-// full lists at cut + skin over bins of half the larger cutoff; neighbour
-// order: stencil bin order, then ascending index within a bin
+// full lists at cut + skin over bins of half the larger cutoff; found in
+// stencil bin order, then ascending index within a bin; the pair list keeps
+// that order, the fix list is sorted by list_sort_key (distance, then index)
 void eph_md::build_lists() {
   const double cf = cfg.cut_fix + cfg.skin, cl = cfg.cut_lj + cfg.skin;
   const double cmax = std::max(cf, cl);
@@ -111,6 +112,7 @@ void eph_md::build_lists() {
   std::vector<int> bins(nall), fillp(bcount.begin(), bcount.end() - 1);
   for (size_t i = 0; i < nall; ++i) bins[fillp[bin_of[i]]++] = (int)i;
   const double cf2 = cf * cf, cl2 = cl * cl;
+  const double sscale = list_sort_scale(cf2);
   std::vector<int> nf(nlocal), nl(nlocal);
   auto scan = [&](long i, bool write, int* of, int* ol) {
     const int bx = bcoord(i, 0), by = bcoord(i, 1), bz = bcoord(i, 2);
@@ -138,7 +140,21 @@ void eph_md::build_lists() {
         }
       }
     }
-    if (!write) { nf[i] = cf_; nl[i] = cl_; }
+    if (!write) {
+      nf[i] = cf_;
+      nl[i] = cl_;
+      return;
+    }
+    std::vector<uint64_t> keys(cf_);
+    const std::vector<int> found(of, of + cf_);
+    for (int t = 0; t < cf_; ++t) {
+      const double* xj = &x[3 * (size_t)found[t]];
+      const double ex = xj[0] - xi[0], ey = xj[1] - xi[1], ez = xj[2] - xi[2];
+      keys[t] = (static_cast<uint64_t>(list_sort_q(ex * ex + ey * ey + ez * ez, sscale)) << 32) |
+                static_cast<uint32_t>(t);
+    }
+    std::sort(keys.begin(), keys.end());
+    for (int t = 0; t < cf_; ++t) of[t] = found[static_cast<uint32_t>(keys[t])];
   };
   parallel_for(nlocal, [&](long i) { scan(i, false, nullptr, nullptr); });
   fix.off.assign(nlocal + 1, 0);
@@ -233,6 +249,7 @@ int eph_md::upload(eph_context* ctx) {
   w.box.lo = box_d.p; w.box.hi = box_d.p + 3; w.box.L = box_d.p + 6; w.box.glo = box_d.p + 9;
   w.box.nb = nbi.p; w.box.bin_base = nbi.p + 3; w.box.bs = bs; w.box.cg = cmax;
   w.lists.nbr_f = ctx->nbr.p; w.lists.nn_f = ctx->nnbr.p; w.lists.cap_f = cap_f; w.lists.cf2 = cf * cf;
+  w.lists.sort_scale = list_sort_scale(w.lists.cf2);
   w.lists.nbr_l = lj_nbr.p; w.lists.nn_l = lj_nn.p; w.lists.cap_l = cap_l; w.lists.cl2 = cl * cl;
   w.lists.stride = ctx->stride;
   w.go = flags.p; w.overflow = flags.p + 1; w.nrebuild = flags.p + 2; w.nghost = flags.p + 4;

@@ -6,8 +6,9 @@
 //     lower image first,
 //   * atoms binned on bins of half the larger list cutoff, each bin holding
 //     its atoms in ascending index,
-//   * full lists in stencil order (z outer, then y, then x, offsets -2..2)
-//     and ascending index within a bin.
+//   * full lists found in stencil order (z outer, then y, then x, offsets
+//     -2..2) and ascending index within a bin; the pair list keeps that
+//     order, the fix list is then sorted by list_sort_key (distance, index).
 // Several independent systems ("replicas") can be built at once: atom i of
 // replica rep[i] only images and neighbours atoms of its own replica.
 // Every kernel first reads the device flag `go` and returns when it is 0, so
@@ -255,6 +256,225 @@ __global__ __launch_bounds__(kBlock) void k_nb_lists(int nlocal, const double4* 
   if (l.cand) l.ncand[i] = nc < l.cap_f ? nc : l.cap_f;
 }
 
+// Each atom's fix list into (q(r^2), found position) order, and its
+// candidates (the entries with r^2 < cand2, in list order) rewritten to
+// match: one wave per atom, a bitonic sort of N = 64 E 32-bit keys q x 2^9 + t
+// (element n = lane E + s; padding keys ~0 sort last; nn_f <= cap_f <= N).
+// The found position t in a key gives the entry's index, read back before the
+// sorted list is written over the found one.  A fixed grid of waves walks the
+// atoms, so a step without a rebuild costs one flag read per block.
+template <int E>
+__device__ __forceinline__ void nb_sort_atom(const int i, const int lane, const double4* __restrict__ pos,
+                                             const NbLists& l) {
+  const int nf = l.nn_f[i];
+  const double4 pi = pos[i];
+  uint32_t key[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int n = lane * E + s;
+    key[s] = ~0u;
+    if (n < nf) {
+      const double4 pj = pos[l.nbr_f[n * l.stride + i]];
+      const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+      key[s] = (list_sort_q(ex * ex + ey * ey + ez * ez, l.sort_scale) << 9) | static_cast<uint32_t>(n);
+    }
+  }
+  constexpr int N = 64 * E;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int d = k >> 1; d > 0; d >>= 1) {
+      if (d < E) {  // both elements of a pair in this lane
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+          const int ps = s ^ d;
+          if (ps > s) {
+            const bool up = ((lane * E + s) & k) == 0;
+            const uint32_t lo = min(key[s], key[ps]), hi = max(key[s], key[ps]);
+            key[s] = up ? lo : hi;
+            key[ps] = up ? hi : lo;
+          }
+        }
+      } else {  // the partner element sits in lane ^ (d / E), same slot
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+          const int n = lane * E + s;
+          const uint32_t mine = key[s];
+          const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(mine), d / E));
+          const bool keep_min = ((n & d) == 0) == ((n & k) == 0);
+          key[s] = keep_min ? min(mine, other) : max(mine, other);
+        }
+      }
+    }
+  }
+  // the indices of the sorted entries, all read before any is written
+  int jv[E];
+#pragma unroll
+  for (int s = 0; s < E; ++s)
+    jv[s] = lane * E + s < nf ? l.nbr_f[static_cast<int>(key[s] & 511u) * l.stride + i] : 0;
+  __builtin_amdgcn_wave_barrier();
+  int mine = 0;  // candidates among this lane's entries
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    const int n = lane * E + s;
+    bool c = false;
+    if (n < nf) {
+      l.nbr_f[n * l.stride + i] = jv[s];
+      if (l.cand) {
+        const double4 pj = pos[jv[s]];
+        const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+        c = ex * ex + ey * ey + ez * ez < l.cand2;
+      }
+    }
+    mine |= (c ? 1 : 0) << s;
+  }
+  if (!l.cand) return;
+  // exclusive prefix of the candidate counts over the lanes
+  const int cnt = __builtin_popcount(mine);
+  int incl = cnt;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const int v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
+  }
+  int at = incl - cnt;
+#pragma unroll
+  for (int s = 0; s < E; ++s) {
+    if ((mine >> s) & 1) {
+      if (at < l.cap_f) l.cand[at * l.stride + i] = jv[s];
+      ++at;
+    }
+  }
+  if (lane == 63) l.ncand[i] = incl < l.cap_f ? incl : l.cap_f;
+}
+
+// The sort network of nb_sort_atom on one wave's E keys per lane
+template <int E>
+__device__ __forceinline__ void bitonic_wave(uint32_t (&key)[E], const int lane) {
+  constexpr int N = 64 * E;
+#pragma unroll
+  for (int k = 2; k <= N; k <<= 1) {
+#pragma unroll
+    for (int d = k >> 1; d > 0; d >>= 1) {
+      if (d < E) {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+          const int ps = s ^ d;
+          if (ps > s) {
+            const bool up = ((lane * E + s) & k) == 0;
+            const uint32_t lo = min(key[s], key[ps]), hi = max(key[s], key[ps]);
+            key[s] = up ? lo : hi;
+            key[ps] = up ? hi : lo;
+          }
+        }
+      } else {
+#pragma unroll
+        for (int s = 0; s < E; ++s) {
+          const int n = lane * E + s;
+          const uint32_t mine = key[s];
+          const uint32_t other = static_cast<uint32_t>(__shfl_xor(static_cast<int>(mine), d / E));
+          const bool keep_min = ((n & d) == 0) == ((n & k) == 0);
+          key[s] = keep_min ? min(mine, other) : max(mine, other);
+        }
+      }
+    }
+  }
+}
+
+// The same sort for lists of at most 256 entries, 64 atoms per block: the
+// list rows of the block's atoms are read and written whole (coalesced)
+// through an LDS tile, and each of the 4 waves sorts 16 of the atoms from it.
+// Rows are padded by one word so that a row-wise and a column-wise pass both
+// avoid bank conflicts.
+template <int E>
+__global__ __launch_bounds__(kBlock) void k_nb_sort_tile(int nlocal, const double4* __restrict__ pos,
+                                                         NbLists l, const int* go) {
+  constexpr int R = 64 * E + 1;
+  __shared__ int s_j[64 * R];        // the block's lists: found order, then sorted
+  __shared__ int s_c[64 * R];        // their candidates
+  __shared__ unsigned char s_cf[64 * 64 * E];  // candidate flag per found position
+  __shared__ int s_nf[64], s_nc[64];
+  if (!*go) return;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  for (int i0 = blockIdx.x * 64; i0 < nlocal; i0 += gridDim.x * 64) {
+    if (threadIdx.x < 64) s_nf[lane] = i0 + lane < nlocal ? l.nn_f[i0 + lane] : 0;
+    __syncthreads();
+    int mx = s_nf[lane];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, __shfl_xor(mx, o));
+    for (int k = wv; k < mx; k += 4)
+      if (k < s_nf[lane]) s_j[lane * R + k] = l.nbr_f[k * l.stride + i0 + lane];
+    __syncthreads();
+    for (int a = wv; a < 64; a += 4) {
+      const int nf = s_nf[a];  // the same for the whole wave
+      if (nf == 0) {
+        if (lane == 0) s_nc[a] = 0;
+        continue;
+      }
+      const int i = i0 + a;
+      const double4 pi = pos[i];
+      uint32_t key[E];
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        const int n = lane * E + s;
+        key[s] = ~0u;
+        if (n < nf) {
+          const double4 pj = pos[s_j[a * R + n]];
+          const double ex = pj.x - pi.x, ey = pj.y - pi.y, ez = pj.z - pi.z;
+          const double r2 = ex * ex + ey * ey + ez * ez;
+          key[s] = (list_sort_q(r2, l.sort_scale) << 9) | static_cast<uint32_t>(n);
+          s_cf[a * 64 * E + n] = r2 < l.cand2 ? 1 : 0;
+        }
+      }
+      bitonic_wave<E>(key, lane);
+      int jv[E];
+      int mine = 0;
+#pragma unroll
+      for (int s = 0; s < E; ++s) {
+        const int t = static_cast<int>(key[s] & 511u);
+        const bool in = lane * E + s < nf;
+        jv[s] = in ? s_j[a * R + t] : 0;
+        mine |= (in && s_cf[a * 64 * E + t] ? 1 : 0) << s;
+      }
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): every read of row a is back
+      __builtin_amdgcn_wave_barrier();
+#pragma unroll
+      for (int s = 0; s < E; ++s)
+        if (lane * E + s < nf) s_j[a * R + lane * E + s] = jv[s];
+      const int cnt = __builtin_popcount(mine);
+      int incl = cnt;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(incl, o);
+        if (lane >= o) incl += v;
+      }
+      int at = incl - cnt;
+#pragma unroll
+      for (int s = 0; s < E; ++s)
+        if ((mine >> s) & 1) s_c[a * R + at++] = jv[s];
+      if (lane == 63) s_nc[a] = incl;
+    }
+    __syncthreads();
+    for (int k = wv; k < mx; k += 4) {
+      if (k < s_nf[lane]) l.nbr_f[k * l.stride + i0 + lane] = s_j[lane * R + k];
+      if (l.cand && k < s_nc[lane]) l.cand[k * l.stride + i0 + lane] = s_c[lane * R + k];
+    }
+    if (l.cand && threadIdx.x < 64 && i0 + lane < nlocal) l.ncand[i0 + lane] = s_nc[lane];
+    __syncthreads();
+  }
+}
+
+constexpr int kSortBlocks = 2048;
+template <int E>
+__global__ __launch_bounds__(kBlock) void k_nb_sort(int nlocal, const double4* __restrict__ pos, NbLists l,
+                                                    const int* go) {
+  if (!*go) return;
+  const int lane = threadIdx.x & 63;
+  const int nw = gridDim.x * (kBlock / 64);
+  for (int i = blockIdx.x * (kBlock / 64) + static_cast<int>(threadIdx.x >> 6); i < nlocal; i += nw)
+    nb_sort_atom<E>(i, lane, pos, l);  // i is the same for the whole wave
+}
+
 __global__ __launch_bounds__(kBlock) void k_nb_finish(int nlocal, const int* ng,
                                                       const double4* __restrict__ pos,
                                                       double4* ref, int* go, int* nreb,
@@ -401,8 +621,17 @@ hipError_t launch_nb_build(const NbBuild& w, hipStream_t s) {
   k_nb_bin_scatter<<<nbk(cap), kBlock, 0, s>>>(cap, w.nlocal, ng, w.bin_of, w.bstart, w.bfill,
                                                w.items, go);
   k_nb_bin_sort<<<nbk(w.nbins), kBlock, 0, s>>>(w.nbins, w.bstart, w.items, go);
+  NbLists found = w.lists;
+  found.cand = nullptr;  // the candidates are written in the sorted order by k_nb_sort
   k_nb_lists<<<nbk(w.nlocal), kBlock, 0, s>>>(w.nlocal, w.at.pos, w.at.rep, w.box, w.bstart,
-                                              w.items, w.lists, w.overflow, go);
+                                              w.items, found, w.overflow, go);
+  const int ek = w.lists.cap_f <= 128 ? 2 : (w.lists.cap_f <= 256 ? 4 : (w.lists.cap_f <= kListSortMax ? 8 : 0));
+  if (ek == 0) return hipErrorInvalidValue;  // more fix-list entries per atom than the sort takes
+  const int sb = std::min(kSortBlocks, nbk(w.nlocal, kBlock / 64));
+  const int tb = std::min(kSortBlocks, nbk(w.nlocal, 64));
+  if (ek == 2) k_nb_sort_tile<2><<<tb, kBlock, 0, s>>>(w.nlocal, w.at.pos, w.lists, go);
+  else if (ek == 4) k_nb_sort_tile<4><<<tb, kBlock, 0, s>>>(w.nlocal, w.at.pos, w.lists, go);
+  else k_nb_sort<8><<<sb, kBlock, 0, s>>>(w.nlocal, w.at.pos, w.lists, go);
   const bool cw = w.lists.cand && w.cref && w.cand_valid;
   k_nb_finish<<<nbk(cw ? cap : w.nlocal), kBlock, 0, s>>>(w.nlocal, ng, w.at.pos, w.xref, w.go,
                                                           w.nrebuild, w.cand_valid, w.cref, cw);

@@ -1,9 +1,28 @@
 // eph_nbuild.hpp -- device rebuild of ghosts and neighbour lists (harness).
 #pragma once
 
+#include <cmath>
+#include <cstdint>
+
 #include "eph_device.hpp"
 
 namespace eph {
+
+// Order of every fix list of the harness (host and device builds alike): the
+// entries are first found in stencil order (found position t), then sorted by
+// (q(r^2), t), q(r^2) = r^2 x scale truncated, r^2 as at the build.  scale is
+// the power of two that keeps cf2 x scale at or below 2^23, so q fits 23 bits
+// and is exact arithmetic on both sides; t breaks ties.  Close pairs come
+// first, and entry k of neighbouring atoms lies at similar distances, which
+// the sweeps' table loads gain from.  The device sorts keys q x 2^9 + t and
+// takes lists of at most kListSortMax entries (a longer list is an error).
+constexpr int kListSortMax = 512;
+inline double list_sort_scale(double cf2) {
+  return std::ldexp(1.0, 23 - static_cast<int>(std::ceil(std::log2(cf2 > 1.0 ? cf2 : 1.0))));
+}
+__host__ __device__ inline uint32_t list_sort_q(double r2, double scale) {
+  return static_cast<uint32_t>(r2 * scale);
+}
 
 // per-replica boxes and bin grids (arrays of 3 per replica unless noted)
 struct NbBoxes {
@@ -31,6 +50,7 @@ struct NbAtoms {
 
 struct NbLists {
   int* nbr_f = nullptr; int* nn_f = nullptr; int cap_f = 0; double cf2 = 0;  // fix list
+  double sort_scale = 0;  // list_sort_scale(cf2): the fix lists' order (see list_sort_key)
   int* nbr_l = nullptr; int* nn_l = nullptr; int cap_l = 0; double cl2 = 0;  // pair list
   int stride = 0;
   // the density sweep's candidates (entries of the fix list with r^2 < cand2,
