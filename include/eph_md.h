@@ -3,7 +3,9 @@
  * pieces FixEPH relies on (no counterpart in USER-EPH itself):
  *   create_atoms fcc + velocity create  -> eph_md_create_fcc
  *   periodic ghost images (Comm::borders, x_ghost = x_owner + shift)
- *   full neighbour list with ghosts at r_cutoff + skin (NeighList)
+ *   full neighbour list with ghosts at r_cutoff + skin (NeighList); the fix's
+ *   list in distance order (squared distance quantized at a power-of-two
+ *   scale, ties in stencil order), host and device builds alike
  *   pair lj/cut (Newton off, full list)
  *   neighbor check yes: rebuild when an atom moved more than skin / 2
  *   Verlet step order: initial_integrate, comm/neighbor, pair, post_force,
