@@ -37,9 +37,12 @@ def _compare_systems(a, b):
     assert np.array_equal(a.nbr_idx, b.nbr_idx)
 
 
-@pytest.mark.parametrize("cells,species", [(6, 1), (9, 4)])
-def test_device_rebuild_matches_host_build(tmp_path, cells, species):
-    cfg = dict(a=3.615, ntypes=species, seed=11, T=2500.0, disp_sigma=0.08)
+# skin 2 A: fix lists of about 90 entries; 3 A: about 145 (the device sort's
+# 256-key LDS-tile form); 5 A: about 320 (512 keys, one wave per atom)
+@pytest.mark.parametrize("cells,species,skin", [(6, 1, 2.0), (9, 4, 2.0), (7, 2, 3.0), (8, 1, 5.0)])
+def test_device_rebuild_matches_host_build(tmp_path, cells, species, skin):
+    print(f"device rebuild: fcc{cells}, {species} species, skin {skin}", flush=True)
+    cfg = dict(a=3.615, ntypes=species, seed=11, T=2500.0, disp_sigma=0.08, skin=skin)
     md_d = ea.MD(cells, **cfg)
     args, inf = _fix(str(tmp_path), md_d, species=species, cells=cells, grid=cells)
     fix = ea.FixEPH(args, species, inf["boxlo"], inf["boxhi"], 1e-3, inputs.BOLTZ, inputs.FTM2V,
@@ -60,6 +63,10 @@ def test_device_rebuild_matches_host_build(tmp_path, cells, species):
     # the device rebuild remaps with the positions it had before the remap;
     # those equal sD's locals after remap, so a second remap is the identity
     _compare_systems(sD, sH)
+    longest = int(np.diff(sD.nbr_off).max())
+    print(f"longest fix list {longest}", flush=True)
+    # list capacity = longest + longest / 4 + 16: skin 3 A within 256 keys, 5 A beyond
+    assert {2.0: True, 3.0: 100 < longest <= 190, 5.0: 200 < longest <= 390}[skin], longest
 
 
 def test_device_resident_run_matches_oracle(oracle, tmp_path):
