@@ -134,7 +134,9 @@ int eph_ensemble::setup(long long first) {
     }
   }
   ghost_cap = ng_est + ng_est / 4 + 1024 * R;
-  cap_f = mxf + mxf / 4 + 16;
+  if (mxf > kListSortMax)
+    return fail(EPH_ERR_UNSUPPORTED, "a fix list holds more than 512 entries (the device list sort's limit)");
+  cap_f = std::min(mxf + mxf / 4 + 16, kListSortMax);
   cap_l = mxl + mxl / 4 + 16;
   // concatenated locals
   std::vector<double> x(3 * (size_t)nl), v(3 * (size_t)nl);

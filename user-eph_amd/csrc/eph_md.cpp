@@ -177,7 +177,10 @@ int eph_md::upload(eph_context* ctx) {
   }
   // capacities for device rebuilds: ghosts and list widths with headroom
   ghost_cap = std::max(ghost_cap, nghost + nghost / 4 + 1024);
-  cap_f = std::max(cap_f, mxf + mxf / 4 + 16);
+  // the device rebuild sorts fix lists of at most kListSortMax entries
+  if (mxf > kListSortMax)
+    return fail(EPH_ERR_UNSUPPORTED, "a fix list holds more than 512 entries (the device list sort's limit)");
+  cap_f = std::min(std::max(cap_f, mxf + mxf / 4 + 16), kListSortMax);
   cap_l = std::max(cap_l, mx + mx / 4 + 16);
   if ((rc = ctx->reserve_atoms(nlocal, ghost_cap))) return fail(rc, ctx->err);
   EPH_HIP(this, ctx->nbr.reserve((size_t)cap_f * ctx->stride));
