@@ -166,6 +166,23 @@ def test_harness_ghosts_lists_and_lattice():
     assert abs(T - 300.0) < 1e-9
 
 
+def test_harness_fix_lists_in_distance_order():
+    """Each fix list of the harness is in ascending q(r^2) = r^2 x scale
+    truncated, scale = 2^(23 - ceil(log2(cf^2))) (ties in stencil order)."""
+    md = ea.MD(5, ntypes=2, seed=8, T=600.0, disp_sigma=0.07)
+    s = md.snapshot()
+    nl = s.nlocal
+    c2 = 6.5 ** 2
+    scale = 2.0 ** (23 - int(np.ceil(np.log2(c2))))
+    for i in range(nl):
+        js = s.nbr_idx[s.nbr_off[i]:s.nbr_off[i + 1]]
+        d = s.x[js] - s.x[i]
+        r2 = (d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1]) + d[:, 2] * d[:, 2]
+        q = np.floor(r2 * scale)
+        assert np.all(q < 2 ** 23)
+        assert np.all(np.diff(q) >= 0), f"atom {i}: fix list not in distance order"
+
+
 def test_harness_lj_forces_match_brute_force():
     md = ea.MD(3, seed=2, T=300.0, disp_sigma=0.1)
     s = md.snapshot()
